@@ -1,0 +1,186 @@
+/*
+ * lair_dev.h -- C ABI of the MI355X (gfx950) LU factor-and-solve library.
+ *
+ * Drop-in boundary for the LU path of vinesystems/lair 0.8.0.  Each entry point
+ * names the reference interface it replaces.  Conventions shared by all calls:
+ *
+ *  * Matrices are described like an ndarray view: a base pointer and element
+ *    strides `rs` (between rows) and `cs` (between columns), which may be
+ *    negative.  The standard (C-order) layout is rs = ncols, cs = 1.
+ *  * Pivots are 0-based absolute row indices in int64 (the reference's
+ *    Vec<usize>): at step i, row i was interchanged with row ipiv[i].  This is
+ *    LAPACK's ipiv minus one.
+ *  * `singular` receives the reference's Option<usize>: -1 for None, otherwise
+ *    the LAST column whose pivot was exactly zero (src/lapack/getrf.rs:72-73).
+ *  * Host-pointer calls are blocking and thread-safe.  `_device` calls take
+ *    device pointers obtained from lair_malloc() and run on the context's
+ *    stream; they return after the work is queued unless stated otherwise.
+ *
+ * Status codes: 0 success; negative -i means argument i was invalid (LAPACK
+ * style); positive values are the LAIR_ERR_* codes below.
+ */
+#ifndef LAIR_DEV_H
+#define LAIR_DEV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t lair_status;
+
+#define LAIR_OK 0
+#define LAIR_ERR_SHAPE 1    /* InvalidInput::Shape (src/lib.rs:26-27) */
+#define LAIR_ERR_SINGULAR 2 /* InvalidInput::Value("`a` is a singular matrix") (src/equation.rs:56) */
+#define LAIR_ERR_DEVICE 3   /* HIP runtime or launch failure (no reference analogue) */
+#define LAIR_ERR_NOMEM 4    /* device allocation failed (no reference analogue) */
+#define LAIR_ERR_NOCONV 5   /* refinement did not reach the tolerance; FP64 fallback was used */
+#define LAIR_ERR_SYNC 6     /* a cooperative kernel's bounded wait expired (no reference analogue) */
+#define LAIR_ERR_UNSUPPORTED 7
+
+typedef struct lair_context lair_context;
+
+/* Precision of the O(n^3) factorization in the mixed solver. */
+#define LAIR_PREC_FP64 0
+#define LAIR_PREC_FP32 1  /* FP32 storage, FP32 MFMA (v_mfma_f32_16x16x4_f32) trailing update */
+#define LAIR_PREC_BF16X6 2 /* FP32 storage, trailing update as six BF16 MFMA products
+                              (3-way bf16 split of each operand), FP32 accumulation */
+#define LAIR_PREC_BF16 3   /* FP32 storage, single BF16 MFMA product (diagnostic; does not
+                              converge on ill-conditioned systems) */
+
+/* ------------------------------------------------------------ context --- */
+
+/* Creates a context on the current HIP device: a stream, workspace and the
+ * autotuner cache.  The host-pointer calls below use a lazily created,
+ * mutex-guarded default context when `ctx` is NULL. */
+lair_status lair_context_create(lair_context **ctx);
+lair_status lair_context_destroy(lair_context *ctx);
+/* Blocks until all work queued on the context's stream has finished. */
+lair_status lair_context_synchronize(lair_context *ctx);
+/* The HIP stream (hipStream_t) the context launches on, as an opaque pointer. */
+void *lair_context_stream(lair_context *ctx);
+/* Human-readable message for the last error seen on this thread. */
+const char *lair_last_error(void);
+
+/* Device memory helpers, so callers need no other GPU runtime. */
+lair_status lair_malloc(void **dptr, uint64_t bytes);
+lair_status lair_free(void *dptr);
+lair_status lair_memcpy_h2d(void *dst, const void *src, uint64_t bytes);
+lair_status lair_memcpy_d2h(void *dst, const void *src, uint64_t bytes);
+lair_status lair_memcpy_d2d(void *dst, const void *src, uint64_t bytes);
+
+/* ------------------------------------------- LAPACK-level entry points --- */
+
+/* Replaces lapack::getrf (src/lapack/getrf.rs:12-27): in-place LU with
+ * partial pivoting of the m x n matrix (a, rs, cs); ipiv receives min(m,n)
+ * pivots and *singular the last zero-pivot column or -1.  Pivots match the
+ * reference's row-major kernel (getrf.rs:46-120) exactly, apart from choices
+ * between values that differ only by rounding. */
+lair_status lair_dgetrf(int64_t m, int64_t n, double *a, int64_t rs, int64_t cs, int64_t *ipiv,
+                        int64_t *singular);
+lair_status lair_sgetrf(int64_t m, int64_t n, float *a, int64_t rs, int64_t cs, int64_t *ipiv,
+                        int64_t *singular);
+
+/* Replaces lapack::getrs (src/lapack/getrs.rs:12-39), extended to nrhs
+ * right-hand sides: solves A X = B (trans = 0) or A^T X = B (trans = 1) from
+ * the packed factors (lu, lrs, lcs) and pivots; b (n x nrhs, strides brs, bcs)
+ * is overwritten with X.  Like the reference, a shape mismatch is the caller's
+ * error; it returns LAIR_ERR_SHAPE here instead of panicking. */
+lair_status lair_dgetrs(int64_t n, int64_t nrhs, const double *lu, int64_t lrs, int64_t lcs,
+                        const int64_t *ipiv, double *b, int64_t brs, int64_t bcs, int32_t trans);
+lair_status lair_sgetrs(int64_t n, int64_t nrhs, const float *lu, int64_t lrs, int64_t lcs,
+                        const int64_t *ipiv, float *b, int64_t brs, int64_t bcs, int32_t trans);
+
+/* Replaces lapack::laswp (src/lapack/laswp.rs:11-40): rows i <-> ipiv[i] for
+ * i in begin..npiv, in forward order, over ncols columns. */
+lair_status lair_dlaswp(int64_t ncols, double *a, int64_t rs, int64_t cs, int64_t begin,
+                        int64_t npiv, const int64_t *ipiv);
+
+/* Replaces equation::solve (src/equation.rs:32-60): x = A^-1 b for square A
+ * (n x n, strides rs, cs) and b of length blen.  Returns LAIR_ERR_SHAPE when A
+ * is not square (m != n) or blen != m, LAIR_ERR_SINGULAR when the
+ * factorization met an exactly zero pivot.  a and b are not modified. */
+lair_status lair_dsolve(int64_t m, int64_t n, const double *a, int64_t rs, int64_t cs,
+                        int64_t blen, const double *b, double *x);
+
+/* ----------------------------------------------- device-resident calls --- */
+
+/* getrf on a device-resident row-major matrix with leading dimension lda.
+ * nb = panel width (0: autotuned).  ipiv is a device int64 array of min(m,n)
+ * entries.  *singular (host) is written after the call completes (the call
+ * synchronizes the stream when singular != NULL). */
+lair_status lair_dgetrf_device(lair_context *ctx, int64_t m, int64_t n, double *a, int64_t lda,
+                               int64_t *ipiv, int64_t nb, int64_t *singular);
+lair_status lair_sgetrf_device(lair_context *ctx, int64_t m, int64_t n, float *a, int64_t lda,
+                               int64_t *ipiv, int64_t nb, int64_t *singular);
+
+/* getrs on device data: b is n x nrhs row-major (leading dimension ldb),
+ * overwritten with the solution of A X = B (trans = 0) or A^T X = B. */
+lair_status lair_dgetrs_device(lair_context *ctx, int64_t n, int64_t nrhs, const double *lu,
+                               int64_t lda, const int64_t *ipiv, double *b, int64_t ldb,
+                               int32_t trans);
+lair_status lair_sgetrs_device(lair_context *ctx, int64_t n, int64_t nrhs, const float *lu,
+                               int64_t lda, const int64_t *ipiv, float *b, int64_t ldb,
+                               int32_t trans);
+
+/* Result record of the mixed-precision solver. */
+typedef struct lair_ir_info {
+  int32_t iterations;   /* refinement steps taken */
+  int32_t converged;    /* 1 when the backward error met the tolerance */
+  int32_t fell_back;    /* 1 when the FP64 factorization was used instead */
+  int32_t precision;    /* LAIR_PREC_* actually used for the final answer */
+  double backward_error;   /* max over columns of ||b-Ax||inf / (||A||inf ||x||inf) */
+  double history[32];      /* backward error after each step (0: initial solve) */
+} lair_ir_info;
+
+/* Mixed-precision solve A X = B from device-resident FP64 data (row-major,
+ * lda, ldb, ldx): the factorization runs in `prec` (LAIR_PREC_FP32 /
+ * LAIR_PREC_BF16X6 / LAIR_PREC_BF16) and FP64 accuracy is recovered by
+ * iterative refinement with the residual, correction update and norms in FP64.
+ * Stops when the normwise backward error is <= tol (tol <= 0 selects
+ * n * DBL_EPSILON) or after max_iter steps; a system that does not converge is
+ * re-solved with the FP64 factorization (info->fell_back = 1, status
+ * LAIR_ERR_NOCONV).  A and B are not modified.  Synchronizes the stream. */
+lair_status lair_dgesv_mixed_device(lair_context *ctx, int64_t n, int64_t nrhs, const double *a,
+                                    int64_t lda, const double *b, int64_t ldb, double *x,
+                                    int64_t ldx, int32_t prec, int32_t max_iter, double tol,
+                                    lair_ir_info *info);
+
+/* FP64 solve A X = B from device data (getrf + getrs), A and B untouched.
+ * The factors are kept in the context's workspace.  Synchronizes the stream
+ * and returns LAIR_ERR_SINGULAR for an exactly singular matrix. */
+lair_status lair_dgesv_device(lair_context *ctx, int64_t n, int64_t nrhs, const double *a,
+                              int64_t lda, const double *b, int64_t ldb, double *x, int64_t ldx);
+
+/* Normwise backward error of X for A X = B, all device FP64 data:
+ * max over columns of ||b - A x||inf / (||A||inf ||x||inf).  Synchronizes. */
+lair_status lair_dbackward_error(lair_context *ctx, int64_t n, int64_t nrhs, const double *a,
+                                 int64_t lda, const double *b, int64_t ldb, const double *x,
+                                 int64_t ldx, double *berr);
+
+/* ----------------------------------------------- instrumentation ------- */
+
+/* Kernel classes timed with HIP events on the context's stream while
+ * profiling is enabled. */
+#define LAIR_KCLASS_PANEL 0
+#define LAIR_KCLASS_LASWP 1
+#define LAIR_KCLASS_TRSM 2
+#define LAIR_KCLASS_GEMM 3
+#define LAIR_KCLASS_GETRS 4
+#define LAIR_KCLASS_RESIDUAL 5
+#define LAIR_KCLASS_CONVERT 6
+#define LAIR_KCLASS_COUNT 7
+
+lair_status lair_profile_enable(lair_context *ctx, int32_t on);
+/* Accumulated milliseconds, launch count, and algorithmic flops for a class
+ * since the last reset (synchronizes the stream). */
+lair_status lair_profile_read(lair_context *ctx, int32_t kclass, double *ms, int64_t *launches,
+                              double *flops);
+lair_status lair_profile_reset(lair_context *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LAIR_DEV_H */

@@ -1,0 +1,279 @@
+"""lair_amd -- MI355X (gfx950) LU factor-and-solve behind vinesystems/lair's interface.
+
+Python mirror of the reference's LU path, calling the C ABI of
+lib/liblair_dev.so (include/lair_dev.h):
+
+* ``getrf(a)``           lapack::getrf      src/lapack/getrf.rs:12-27
+* ``getrs(a, p, b)``     lapack::getrs      src/lapack/getrs.rs:12-39
+* ``laswp(...)``         lapack::laswp      src/lapack/laswp.rs:11-40
+* ``equation.solve``     equation::solve    src/equation.rs:32-60
+* ``lu.Factorized``      decomposition::lu  src/decomposition/lu.rs:12-171
+
+Arrays are numpy views; their strides are passed through unchanged (negative
+strides and column-major views are accepted like ndarray views).  Every call
+runs on the GPU; there is no CPU fallback.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _capi
+from ._capi import LairError, check
+
+__all__ = ["getrf", "getrs", "laswp", "InvalidInput", "ShapeError", "ValueError_", "equation", "lu",
+           "Context", "DeviceArray", "LairError"]
+
+
+class InvalidInput(Exception):
+    """src/lib.rs:23-30."""
+
+
+class ShapeError(InvalidInput):
+    """InvalidInput::Shape."""
+
+
+class ValueError_(InvalidInput):
+    """InvalidInput::Value."""
+
+
+def _elem_strides(a):
+    return a.strides[0] // a.itemsize, a.strides[1] // a.itemsize
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def getrf(a):
+    """In-place LU with partial pivoting of a 2-D float64/float32 view.
+
+    Returns ``(pivots, singular)``: int64 pivots of length min(m, n) (row i was
+    interchanged with row pivots[i]) and the last zero-pivot column or None,
+    exactly as lapack::getrf (src/lapack/getrf.rs:12-27)."""
+    lib = _capi.load()
+    if a.ndim != 2:
+        raise ValueError("getrf expects a 2-D array")
+    m, n = a.shape
+    rs, cs = _elem_strides(a)
+    piv = np.zeros(min(m, n), np.int64)
+    sing = ctypes.c_int64(-1)
+    if a.dtype == np.float64:
+        fn = lib.lair_dgetrf
+    elif a.dtype == np.float32:
+        fn = lib.lair_sgetrf
+    else:
+        raise TypeError("getrf supports float64 and float32")
+    check(fn(m, n, _ptr(a), rs, cs, piv.ctypes.data_as(_capi.P64), ctypes.byref(sing)), "getrf")
+    return piv, (None if sing.value < 0 else int(sing.value))
+
+
+def getrs(a, p, b, trans=False):
+    """Solves a x = b from the packed factors and pivots (src/lapack/getrs.rs:12-39).
+
+    ``b`` may be 1-D (the reference's only case) or 2-D with one right-hand
+    side per column; a new array is returned.  Shape mismatches raise
+    ValueError where the reference panics (getrs.rs:18-20)."""
+    lib = _capi.load()
+    p = np.ascontiguousarray(p, np.int64)
+    if a.shape[0] != len(p) or b.shape[0] != len(p) or a.shape[1] < len(p):
+        raise ValueError("getrs: shape mismatch between a, p and b")
+    n = len(p)
+    x = np.array(b, dtype=a.dtype, copy=True)
+    x2 = x.reshape(n, -1)
+    if a.dtype == np.float64:
+        fn = lib.lair_dgetrs
+    elif a.dtype == np.float32:
+        fn = lib.lair_sgetrs
+    else:
+        raise TypeError("getrs supports float64 and float32")
+    lrs, lcs = _elem_strides(a)
+    brs, bcs = _elem_strides(x2)
+    check(fn(n, x2.shape[1], _ptr(a), lrs, lcs, p.ctypes.data_as(_capi.P64), _ptr(x2), brs, bcs,
+             1 if trans else 0), "getrs")
+    return x
+
+
+def laswp(ncols, a, begin, piv):
+    """Row interchanges rows i <-> piv[i] for i in begin..len(piv), forward
+    order, over ncols columns of the float64 view a (src/lapack/laswp.rs:11-40)."""
+    lib = _capi.load()
+    piv = np.ascontiguousarray(piv, np.int64)
+    rs, cs = _elem_strides(a)
+    check(lib.lair_dlaswp(ncols, _ptr(a), rs, cs, begin, len(piv), piv.ctypes.data_as(_capi.P64)),
+          "laswp")
+
+
+class equation:
+    """src/equation.rs"""
+
+    @staticmethod
+    def solve(a, b):
+        """x = a^-1 b; raises ShapeError / ValueError_ like InvalidInput::{Shape, Value}."""
+        lib = _capi.load()
+        a = np.asarray(a, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        if a.ndim != 2 or b.ndim != 1:
+            raise ShapeError("a must be 2-D and b 1-D")
+        m, n = a.shape
+        rs, cs = _elem_strides(a)
+        x = np.zeros(max(n, 1), np.float64)
+        st = lib.lair_dsolve(m, n, _ptr(a), rs, cs, len(b), _ptr(b), _ptr(x))
+        if st == _capi.LAIR_ERR_SHAPE:
+            raise ShapeError(_capi.last_error())
+        if st == _capi.LAIR_ERR_SINGULAR:
+            raise ValueError_("`a` is a singular matrix")
+        check(st, "solve")
+        return x[:n]
+
+
+class lu:
+    """src/decomposition/lu.rs"""
+
+    class Factorized:
+        """LU decomposition factors (lu.rs:12-20); construction factors a copy
+        of the matrix on the GPU (lu.rs:156-171)."""
+
+        def __init__(self, a):
+            self.lu = np.array(a, copy=True)
+            self.pivots, self.singular = getrf(self.lu)
+
+        def p(self):
+            """lu.rs:28-39: P with P[perm[i], i] = 1, perm = laswp of 0..nrows."""
+            nr = self.lu.shape[0]
+            perm = np.arange(nr, dtype=np.int64)
+            for i, piv in enumerate(self.pivots):
+                if piv != i:
+                    perm[i], perm[piv] = perm[piv], perm[i]
+            p = np.zeros((nr, nr), self.lu.dtype)
+            p[perm, np.arange(nr)] = 1
+            return p
+
+        def l(self):
+            """lu.rs:42-57: unit lower nrows x rank."""
+            nr, nc = self.lu.shape
+            rank = min(nr, nc)
+            l = np.tril(self.lu[:, :rank], -1)
+            l[np.arange(rank), np.arange(rank)] = 1
+            return l
+
+        def u(self):
+            """lu.rs:60-72: upper rank x ncols."""
+            nr, nc = self.lu.shape
+            rank = min(nr, nc)
+            return np.triu(self.lu[:rank, :])
+
+        def is_singular(self):
+            return self.singular is not None
+
+        def solve(self, b):
+            """lu.rs:87-98."""
+            if b.shape[0] != self.lu.shape[0]:
+                raise ShapeError(f"b must have {self.lu.shape[0]} elements")
+            return getrs(self.lu, self.pivots, b)
+
+        def into_pl(self):
+            """lu.rs:107-153: P*L in the first min(m,n) columns, in place."""
+            nr, nc = self.lu.shape
+            k = min(nr, nc)
+            pivots = list(self.pivots) + list(range(len(self.pivots), nr))
+            for i in range(len(pivots) - 1, -1, -1):
+                target = pivots[i]
+                if i == target:
+                    continue
+                pivots[i] = pivots[target]
+                pivots[target] = i
+            pl = self.lu[:, :k]
+            total = len(pivots)
+            dst = 0
+            i = dst
+            while True:
+                src = pivots[dst]
+                for c in range(min(src, k)):
+                    pl[dst, c] = pl[src, c]
+                if src < k:
+                    pl[dst, src] = 1
+                for c in range(src + 1, k):
+                    pl[dst, c] = 0
+                pivots[dst] = total
+                if pivots[src] == total:
+                    dst = i + 1
+                    while dst < total and pivots[dst] == total:
+                        dst += 1
+                    if dst == total:
+                        break
+                    i = dst
+                else:
+                    dst = src
+            return self.lu
+
+
+class Context:
+    """A device context (stream + workspace) of liblair_dev.so."""
+
+    def __init__(self):
+        self.lib = _capi.load()
+        h = ctypes.c_void_p()
+        check(self.lib.lair_context_create(ctypes.byref(h)), "context_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.lair_context_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def synchronize(self):
+        check(self.lib.lair_context_synchronize(self.h), "synchronize")
+
+
+class DeviceArray:
+    """A row-major device buffer (rows x cols, leading dimension ld elements)."""
+
+    def __init__(self, rows, cols, dtype=np.float64, ld=None):
+        self.lib = _capi.load()
+        self.rows, self.cols = rows, cols
+        self.dtype = np.dtype(dtype)
+        self.ld = ld if ld is not None else cols
+        self.nbytes = max(rows * self.ld * self.dtype.itemsize, 1)
+        p = ctypes.c_void_p()
+        check(self.lib.lair_malloc(ctypes.byref(p), self.nbytes), "malloc")
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, a, ld=None):
+        a = np.asarray(a)
+        if a.ndim == 1:
+            a = a.reshape(-1, 1)
+        d = cls(a.shape[0], a.shape[1], a.dtype, ld)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.asarray(a, self.dtype).reshape(self.rows, self.cols)
+        if self.ld == self.cols:
+            h = np.ascontiguousarray(a)
+        else:
+            h = np.zeros((self.rows, self.ld), self.dtype)
+            h[:, :self.cols] = a
+        check(self.lib.lair_memcpy_h2d(self.ptr, ctypes.c_void_p(h.ctypes.data), h.nbytes), "h2d")
+
+    def download(self):
+        h = np.empty((self.rows, self.ld), self.dtype)
+        check(self.lib.lair_memcpy_d2h(ctypes.c_void_p(h.ctypes.data), self.ptr, h.nbytes), "d2h")
+        return np.ascontiguousarray(h[:, :self.cols])
+
+    def free(self):
+        if self.ptr:
+            self.lib.lair_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -1,0 +1,788 @@
+// Host driver of the gfx950 LU path: context, blocked factorization loop,
+// triangular solves, mixed-precision refinement and the C ABI.
+//
+// Reference interfaces mirrored here (vinesystems/lair 0.8.0):
+//   lapack::getrf   src/lapack/getrf.rs:12-27   -> lair_dgetrf / lair_sgetrf
+//   lapack::getrs   src/lapack/getrs.rs:12-39   -> lair_dgetrs / lair_sgetrs
+//   lapack::laswp   src/lapack/laswp.rs:11-40   -> lair_dlaswp
+//   equation::solve src/equation.rs:32-60       -> lair_dsolve
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+
+#include "lair_internal.h"
+
+namespace lair {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+#define LAIR_HIP(call)                                                                 \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      set_error(std::string(#call) + ": " + hipGetErrorString(e_));                    \
+      return e_ == hipErrorOutOfMemory ? LAIR_ERR_NOMEM : LAIR_ERR_DEVICE;             \
+    }                                                                                  \
+  } while (0)
+
+hipError_t ensure(Buffer &b, size_t bytes) {
+  if (b.bytes >= bytes && b.ptr) return hipSuccess;
+  if (b.ptr) {
+    hipError_t e = hipFree(b.ptr);
+    if (e != hipSuccess) return e;
+    b.ptr = nullptr;
+    b.bytes = 0;
+  }
+  size_t want = std::max<size_t>(bytes, 256);
+  hipError_t e = hipMalloc(&b.ptr, want);
+  if (e != hipSuccess) {
+    b.ptr = nullptr;
+    return e;
+  }
+  b.bytes = want;
+  return hipSuccess;
+}
+
+int prof_begin(lair_context *ctx) {
+  if (!ctx->profiling) return -1;
+  if (ctx->event_next + 2 > ctx->event_pool.size()) {
+    for (int i = 0; i < 64; ++i) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return -1;
+      ctx->event_pool.push_back(e);
+    }
+  }
+  const int slot = (int)ctx->event_next;
+  ctx->event_next += 2;
+  hipEventRecord(ctx->event_pool[slot], ctx->stream);
+  return slot;
+}
+
+void prof_end(lair_context *ctx, int slot, int kclass, double flops) {
+  if (slot < 0) return;
+  hipEventRecord(ctx->event_pool[slot + 1], ctx->stream);
+  ctx->prof.push_back({kclass, flops, ctx->event_pool[slot], ctx->event_pool[slot + 1]});
+}
+
+static void prof_collect(lair_context *ctx) {
+  if (ctx->prof.empty()) return;
+  hipStreamSynchronize(ctx->stream);
+  for (auto &r : ctx->prof) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.start, r.stop) == hipSuccess) ctx->prof_ms[r.kclass] += ms;
+    ctx->prof_n[r.kclass] += 1;
+    ctx->prof_flops[r.kclass] += r.flops;
+  }
+  ctx->prof.clear();
+  ctx->event_next = 0;
+}
+
+static int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+// Leading dimension for device copies: a multiple of 64 elements, nudged off
+// multiples of 4096 elements so rows do not all start on the same channel.
+template <typename T>
+static int64_t pad_ld(int64_t n) {
+  int64_t ld = round_up(std::max<int64_t>(n, 1), 64);
+  if (ld % 4096 == 0) ld += 64;
+  return ld;
+}
+
+static int choose_nb(int64_t m, int64_t n) {
+  const int64_t k = std::min(m, n);
+  if (k <= 256) return 64;
+  return 128;
+}
+
+static lair_status hip_status(hipError_t e, const char *what) {
+  if (e == hipSuccess) return LAIR_OK;
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return e == hipErrorOutOfMemory ? LAIR_ERR_NOMEM : LAIR_ERR_DEVICE;
+}
+
+// Blocked right-looking LU with partial pivoting (the panel -> laswp -> trsm
+// -> gemm sequence of the reference's recursive_inner, getrf.rs:216-322, with
+// a fixed panel width nb).  Each panel of nb columns is factored as leaves of
+// kLeafW columns with exact partial pivoting; the leaf's interchanges are
+// applied to the rest of the panel, the leaf's U rows solved and the rest of
+// the panel updated before the next leaf.  After the panel, its interchanges
+// are applied to the columns left and right of it, U12 = L11^-1 A12 and
+// A22 -= L21 U12.
+template <typename T>
+static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a, int64_t lda,
+                              int64_t *ipiv, int64_t nb, int mode, int64_t *singular) {
+  if (m64 < 0) return -2;
+  if (n64 < 0) return -3;
+  if (lda < std::max<int64_t>(n64, 1)) return -5;
+  if (m64 > (int64_t)kMaxPanelWG * 1024 || n64 > INT32_MAX) {
+    set_error("matrix too large for the cooperative panel");
+    return LAIR_ERR_UNSUPPORTED;
+  }
+  const int m = (int)m64, n = (int)n64;
+  const int kmax = std::min(m, n);
+  if (nb <= 0) nb = choose_nb(m, n);
+  if (nb > 256) nb = 256;
+  if (nb < kLeafW) nb = kLeafW;
+  nb = (nb / kLeafW) * kLeafW;
+  hipError_t e = launch_fill_i64(ctx, ctx->sing, 1, -1);
+  if (e != hipSuccess) return hip_status(e, "fill");
+  // Keep sequence numbers far from wrap-around.
+  if (ctx->seq > 0x70000000u) {
+    e = launch_fill_u32(ctx, ctx->flags, kMaxPanelWG, 0u);
+    if (e != hipSuccess) return hip_status(e, "fill");
+    ctx->seq = 0;
+  }
+  for (int k = 0; k < kmax; k += (int)nb) {
+    const int kb = std::min<int>((int)nb, kmax - k);
+    for (int c = k; c < k + kb; c += kLeafW) {
+      const int w = std::min(kLeafW, k + kb - c);
+      e = launch_panel<T>(ctx, a, lda, c, m, c, w, ipiv);
+      if (e != hipSuccess) return hip_status(e, "panel");
+      if (kb > w) {
+        e = launch_laswp<T>(ctx, a, lda, k, k + kb, c, c + w, ipiv, c, c + w, false);
+        if (e != hipSuccess) return hip_status(e, "laswp");
+      }
+      const int rest = k + kb - (c + w);
+      if (rest > 0) {
+        e = launch_trsm<T>(ctx, true, false, true, w, rest, a + (int64_t)c * lda + c, lda,
+                           a + (int64_t)c * lda + c + w, lda);
+        if (e != hipSuccess) return hip_status(e, "trsm");
+        e = launch_gemm_minus<T>(ctx, mode, m - (c + w), rest, w, a + (int64_t)(c + w) * lda + c,
+                                 lda, false, a + (int64_t)c * lda + c + w, lda,
+                                 a + (int64_t)(c + w) * lda + c + w, lda);
+        if (e != hipSuccess) return hip_status(e, "gemm");
+      }
+    }
+    e = launch_laswp<T>(ctx, a, lda, 0, n, k, k + kb, ipiv, k, k + kb, false);
+    if (e != hipSuccess) return hip_status(e, "laswp");
+    const int right = n - (k + kb);
+    if (right > 0) {
+      e = launch_trsm<T>(ctx, true, false, true, kb, right, a + (int64_t)k * lda + k, lda,
+                         a + (int64_t)k * lda + k + kb, lda);
+      if (e != hipSuccess) return hip_status(e, "trsm");
+      if (m - (k + kb) > 0) {
+        e = launch_gemm_minus<T>(ctx, mode, m - (k + kb), right, kb, a + (int64_t)(k + kb) * lda + k,
+                                 lda, false, a + (int64_t)k * lda + k + kb, lda,
+                                 a + (int64_t)(k + kb) * lda + k + kb, lda);
+        if (e != hipSuccess) return hip_status(e, "gemm");
+      }
+    }
+  }
+  if (singular) {
+    int32_t err = 0;
+    LAIR_HIP(hipMemcpyAsync(singular, ctx->sing, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    LAIR_HIP(hipMemcpyAsync(&err, ctx->err, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    LAIR_HIP(hipStreamSynchronize(ctx->stream));
+    if (err) {
+      set_error("cooperative panel wait expired");
+      return LAIR_ERR_SYNC;
+    }
+  }
+  return LAIR_OK;
+}
+
+// Blocked getrs (getrs.rs:12-39 generalized to nrhs columns and to A^T):
+//   A X = B   : forward laswp, L (unit) forward blocks, U backward blocks;
+//   A^T X = B : U^T forward blocks, L^T (unit) backward blocks, reverse laswp.
+template <typename T>
+static lair_status getrs_impl(lair_context *ctx, int64_t n64, int64_t nrhs64, const T *lu,
+                              int64_t lda, const int64_t *ipiv, T *b, int64_t ldb, int trans) {
+  if (n64 < 0) return -2;
+  if (nrhs64 < 0) return -3;
+  if (n64 == 0 || nrhs64 == 0) return LAIR_OK;
+  const int n = (int)n64, nrhs = (int)nrhs64;
+  const int bs = 128;
+  hipError_t e;
+  const int mode = std::is_same<T, double>::value ? kGemmF64 : kGemmF32;
+  if (!trans && n <= kExactOrderMax) {
+    e = launch_getrs_exact<T>(ctx, n, nrhs, lu, lda, ipiv, b, ldb);
+    return hip_status(e, "getrs");
+  }
+  if (!trans) {
+    e = launch_laswp<T>(ctx, b, ldb, 0, nrhs, 0, 0, ipiv, 0, n, false);
+    if (e != hipSuccess) return hip_status(e, "laswp");
+    for (int k = 0; k < n; k += bs) {
+      const int kb = std::min(bs, n - k);
+      e = launch_trsm<T>(ctx, true, false, true, kb, nrhs, lu + (int64_t)k * lda + k, lda,
+                         b + (int64_t)k * ldb, ldb);
+      if (e != hipSuccess) return hip_status(e, "trsm");
+      if (k + kb < n) {
+        e = launch_gemm_minus<T>(ctx, mode, n - k - kb, nrhs, kb, lu + (int64_t)(k + kb) * lda + k,
+                                 lda, false, b + (int64_t)k * ldb, ldb, b + (int64_t)(k + kb) * ldb,
+                                 ldb);
+        if (e != hipSuccess) return hip_status(e, "gemm");
+      }
+    }
+    const int last = ((n - 1) / bs) * bs;
+    for (int k = last; k >= 0; k -= bs) {
+      const int kb = std::min(bs, n - k);
+      e = launch_trsm<T>(ctx, false, false, false, kb, nrhs, lu + (int64_t)k * lda + k, lda,
+                         b + (int64_t)k * ldb, ldb);
+      if (e != hipSuccess) return hip_status(e, "trsm");
+      if (k > 0) {
+        e = launch_gemm_minus<T>(ctx, mode, k, nrhs, kb, lu + k, lda, false, b + (int64_t)k * ldb,
+                                 ldb, b, ldb);
+        if (e != hipSuccess) return hip_status(e, "gemm");
+      }
+    }
+  } else {
+    for (int k = 0; k < n; k += bs) {
+      const int kb = std::min(bs, n - k);
+      e = launch_trsm<T>(ctx, false, true, false, kb, nrhs, lu + (int64_t)k * lda + k, lda,
+                         b + (int64_t)k * ldb, ldb);
+      if (e != hipSuccess) return hip_status(e, "trsm");
+      if (k + kb < n) {
+        // rows k+kb.. of U^T = (U[k..k+kb, k+kb..n])^T, stored K x M at lu[k][k+kb]
+        e = launch_gemm_minus<T>(ctx, mode, n - k - kb, nrhs, kb, lu + (int64_t)k * lda + k + kb,
+                                 lda, true, b + (int64_t)k * ldb, ldb, b + (int64_t)(k + kb) * ldb,
+                                 ldb);
+        if (e != hipSuccess) return hip_status(e, "gemm");
+      }
+    }
+    const int last = ((n - 1) / bs) * bs;
+    for (int k = last; k >= 0; k -= bs) {
+      const int kb = std::min(bs, n - k);
+      e = launch_trsm<T>(ctx, true, true, true, kb, nrhs, lu + (int64_t)k * lda + k, lda,
+                         b + (int64_t)k * ldb, ldb);
+      if (e != hipSuccess) return hip_status(e, "trsm");
+      if (k > 0) {
+        // rows 0..k of L^T = (L[k..k+kb, 0..k])^T, stored K x M at lu[k][0]
+        e = launch_gemm_minus<T>(ctx, mode, k, nrhs, kb, lu + (int64_t)k * lda, lda, true,
+                                 b + (int64_t)k * ldb, ldb, b, ldb);
+        if (e != hipSuccess) return hip_status(e, "gemm");
+      }
+    }
+    e = launch_laswp<T>(ctx, b, ldb, 0, nrhs, 0, 0, ipiv, 0, n, true);
+    if (e != hipSuccess) return hip_status(e, "laswp");
+  }
+  return LAIR_OK;
+}
+
+static double bits_to_double(unsigned long long b) {
+  double d;
+  std::memcpy(&d, &b, sizeof d);
+  return d;
+}
+
+// max over columns of ||b - A x||inf / (||A||inf ||x||inf) with r = b - Ax
+// already in r.  norms must hold 1 + 2*nrhs words; norms[0] = ||A||inf bits.
+static lair_status backward_error_from(lair_context *ctx, int n, int nrhs, const double *r,
+                                       int64_t ldr, const double *x, int64_t ldx,
+                                       unsigned long long *norms, double *berr) {
+  hipError_t e = launch_fill_f64(ctx, reinterpret_cast<double *>(norms + 1), 2 * nrhs, 0.0);
+  if (e != hipSuccess) return hip_status(e, "fill");
+  e = launch_colmax(ctx, n, nrhs, r, ldr, norms + 1);
+  if (e != hipSuccess) return hip_status(e, "colmax");
+  e = launch_colmax(ctx, n, nrhs, x, ldx, norms + 1 + nrhs);
+  if (e != hipSuccess) return hip_status(e, "colmax");
+  std::vector<unsigned long long> h(1 + 2 * (size_t)nrhs);
+  LAIR_HIP(hipMemcpyAsync(h.data(), norms, h.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+  LAIR_HIP(hipStreamSynchronize(ctx->stream));
+  const double na = bits_to_double(h[0]);
+  double worst = 0.0;
+  for (int c = 0; c < nrhs; ++c) {
+    const double nr = bits_to_double(h[1 + c]);
+    const double nx = bits_to_double(h[1 + nrhs + c]);
+    double be;
+    if (nr == 0.0)
+      be = 0.0;
+    else if (na == 0.0 || nx == 0.0)
+      be = INFINITY;
+    else
+      be = nr / (na * nx);
+    if (!(be <= worst)) worst = be;  // NaN propagates
+  }
+  *berr = worst;
+  return LAIR_OK;
+}
+
+}  // namespace lair
+
+using namespace lair;
+
+// ------------------------------------------------------------------ C ABI
+
+static std::mutex g_default_mu;
+static lair_context *g_default = nullptr;
+
+static lair_context *default_ctx() {
+  std::lock_guard<std::mutex> lk(g_default_mu);
+  if (!g_default) {
+    if (lair_context_create(&g_default) != LAIR_OK) g_default = nullptr;
+  }
+  return g_default;
+}
+
+// ---- host-pointer entry points ------------------------------------------
+
+template <typename T>
+static void gather(int64_t m, int64_t n, const T *a, int64_t rs, int64_t cs, T *dst, int64_t ld) {
+  for (int64_t i = 0; i < m; ++i)
+    for (int64_t j = 0; j < n; ++j) dst[i * ld + j] = a[i * rs + j * cs];
+}
+template <typename T>
+static void scatter(int64_t m, int64_t n, const T *src, int64_t ld, T *a, int64_t rs, int64_t cs) {
+  for (int64_t i = 0; i < m; ++i)
+    for (int64_t j = 0; j < n; ++j) a[i * rs + j * cs] = src[i * ld + j];
+}
+
+template <typename T>
+static lair_status host_getrf(int64_t m, int64_t n, T *a, int64_t rs, int64_t cs, int64_t *ipiv,
+                              int64_t *singular) {
+  if (m < 0) return -1;
+  if (n < 0) return -2;
+  if (m > 0 && n > 0 && !a) return -3;
+  const int64_t kmin = std::min(m, n);
+  if (kmin > 0 && !ipiv) return -6;
+  if (!singular) return -7;
+  *singular = -1;
+  if (m == 0 || n == 0) return LAIR_OK;
+  lair_context *ctx = default_ctx();
+  if (!ctx) return LAIR_ERR_DEVICE;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  const int64_t ld = pad_ld<T>(n);
+  std::vector<T> h((size_t)(m * ld));
+  gather(m, n, a, rs, cs, h.data(), ld);
+  LAIR_HIP(ensure(ctx->ws_tmp, sizeof(T) * m * ld));
+  LAIR_HIP(ensure(ctx->ws_ipiv, sizeof(int64_t) * kmin));
+  T *da = static_cast<T *>(ctx->ws_tmp.ptr);
+  int64_t *dp = static_cast<int64_t *>(ctx->ws_ipiv.ptr);
+  LAIR_HIP(hipMemcpyAsync(da, h.data(), sizeof(T) * m * ld, hipMemcpyHostToDevice, ctx->stream));
+  lair_status st = getrf_impl<T>(ctx, m, n, da, ld, dp, 0,
+                                 std::is_same<T, double>::value ? kGemmF64 : kGemmF32, singular);
+  if (st != LAIR_OK) return st;
+  LAIR_HIP(hipMemcpyAsync(h.data(), da, sizeof(T) * m * ld, hipMemcpyDeviceToHost, ctx->stream));
+  LAIR_HIP(hipMemcpyAsync(ipiv, dp, sizeof(int64_t) * kmin, hipMemcpyDeviceToHost, ctx->stream));
+  LAIR_HIP(hipStreamSynchronize(ctx->stream));
+  scatter(m, n, h.data(), ld, a, rs, cs);
+  return LAIR_OK;
+}
+
+
+template <typename T>
+static lair_status host_getrs(int64_t n, int64_t nrhs, const T *lu, int64_t lrs, int64_t lcs,
+                              const int64_t *ipiv, T *b, int64_t brs, int64_t bcs, int32_t trans) {
+  if (n < 0) return -1;
+  if (nrhs < 0) return -2;
+  if (n > 0 && (!lu || !ipiv)) return -3;
+  for (int64_t i = 0; i < n; ++i)
+    if (ipiv[i] < 0 || ipiv[i] >= n) {
+      set_error("pivot index out of range");
+      return LAIR_ERR_SHAPE;
+    }
+  if (n == 0 || nrhs == 0) return LAIR_OK;
+  lair_context *ctx = default_ctx();
+  if (!ctx) return LAIR_ERR_DEVICE;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  const int64_t ld = pad_ld<T>(n);
+  std::vector<T> h((size_t)(n * ld)), hb((size_t)(n * nrhs));
+  gather(n, n, lu, lrs, lcs, h.data(), ld);
+  gather(n, nrhs, b, brs, bcs, hb.data(), nrhs);
+  LAIR_HIP(ensure(ctx->ws_tmp, sizeof(T) * n * ld));
+  LAIR_HIP(ensure(ctx->ws_rhs, sizeof(T) * n * nrhs));
+  LAIR_HIP(ensure(ctx->ws_ipiv, sizeof(int64_t) * n));
+  T *da = static_cast<T *>(ctx->ws_tmp.ptr);
+  T *db = static_cast<T *>(ctx->ws_rhs.ptr);
+  int64_t *dp = static_cast<int64_t *>(ctx->ws_ipiv.ptr);
+  LAIR_HIP(hipMemcpyAsync(da, h.data(), sizeof(T) * n * ld, hipMemcpyHostToDevice, ctx->stream));
+  LAIR_HIP(hipMemcpyAsync(db, hb.data(), sizeof(T) * n * nrhs, hipMemcpyHostToDevice, ctx->stream));
+  LAIR_HIP(hipMemcpyAsync(dp, ipiv, sizeof(int64_t) * n, hipMemcpyHostToDevice, ctx->stream));
+  lair_status st = getrs_impl<T>(ctx, n, nrhs, da, ld, dp, db, nrhs, trans);
+  if (st != LAIR_OK) return st;
+  LAIR_HIP(hipMemcpyAsync(hb.data(), db, sizeof(T) * n * nrhs, hipMemcpyDeviceToHost, ctx->stream));
+  LAIR_HIP(hipStreamSynchronize(ctx->stream));
+  scatter(n, nrhs, hb.data(), nrhs, b, brs, bcs);
+  return LAIR_OK;
+}
+
+
+extern "C" {
+
+const char *lair_last_error(void) { return g_last_error.c_str(); }
+
+lair_status lair_context_create(lair_context **out) {
+  if (!out) return -1;
+  std::unique_ptr<lair_context> ctx(new lair_context());
+  LAIR_HIP(hipGetDevice(&ctx->device));
+  LAIR_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  LAIR_HIP(hipMalloc(&ctx->flags, sizeof(uint32_t) * kMaxPanelWG));
+  LAIR_HIP(hipMalloc(&ctx->cand_val, sizeof(double) * 2 * kMaxPanelWG));
+  LAIR_HIP(hipMalloc(&ctx->cand_idx, sizeof(int32_t) * 2 * kMaxPanelWG));
+  LAIR_HIP(hipMalloc(&ctx->cand_row, sizeof(double) * 2 * kMaxPanelWG * kLeafW));
+  LAIR_HIP(hipMalloc(&ctx->diag_row, sizeof(double) * 2 * kLeafW));
+  LAIR_HIP(hipMalloc(&ctx->err, sizeof(int32_t)));
+  LAIR_HIP(hipMalloc(&ctx->sing, sizeof(int64_t)));
+  LAIR_HIP(hipMalloc(&ctx->norms, sizeof(double) * 4096));
+  LAIR_HIP(launch_fill_u32(ctx.get(), ctx->flags, kMaxPanelWG, 0u));
+  LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->err), 1, 0u));
+  LAIR_HIP(hipStreamSynchronize(ctx->stream));
+  *out = ctx.release();
+  return LAIR_OK;
+}
+
+lair_status lair_context_destroy(lair_context *ctx) {
+  if (!ctx) return -1;
+  hipStreamSynchronize(ctx->stream);
+  for (auto e : ctx->event_pool) hipEventDestroy(e);
+  hipFree(ctx->flags);
+  hipFree(ctx->cand_val);
+  hipFree(ctx->cand_idx);
+  hipFree(ctx->cand_row);
+  hipFree(ctx->diag_row);
+  hipFree(ctx->err);
+  hipFree(ctx->sing);
+  hipFree(ctx->norms);
+  for (Buffer *b : {&ctx->ws_a32, &ctx->ws_lu64, &ctx->ws_ipiv, &ctx->ws_x32, &ctx->ws_r,
+                    &ctx->ws_tmp, &ctx->ws_rhs})
+    if (b->ptr) hipFree(b->ptr);
+  hipStreamDestroy(ctx->stream);
+  if (ctx == g_default) g_default = nullptr;
+  delete ctx;
+  return LAIR_OK;
+}
+
+lair_status lair_context_synchronize(lair_context *ctx) {
+  if (!ctx) ctx = default_ctx();
+  if (!ctx) return LAIR_ERR_DEVICE;
+  LAIR_HIP(hipStreamSynchronize(ctx->stream));
+  return LAIR_OK;
+}
+
+void *lair_context_stream(lair_context *ctx) {
+  if (!ctx) ctx = default_ctx();
+  return ctx ? (void *)ctx->stream : nullptr;
+}
+
+lair_status lair_malloc(void **dptr, uint64_t bytes) {
+  if (!dptr) return -1;
+  LAIR_HIP(hipMalloc(dptr, bytes ? bytes : 1));
+  return LAIR_OK;
+}
+lair_status lair_free(void *dptr) {
+  LAIR_HIP(hipFree(dptr));
+  return LAIR_OK;
+}
+lair_status lair_memcpy_h2d(void *dst, const void *src, uint64_t bytes) {
+  LAIR_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return LAIR_OK;
+}
+lair_status lair_memcpy_d2h(void *dst, const void *src, uint64_t bytes) {
+  LAIR_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return LAIR_OK;
+}
+lair_status lair_memcpy_d2d(void *dst, const void *src, uint64_t bytes) {
+  LAIR_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  return LAIR_OK;
+}
+
+lair_status lair_dgetrf_device(lair_context *ctx, int64_t m, int64_t n, double *a, int64_t lda,
+                               int64_t *ipiv, int64_t nb, int64_t *singular) {
+  if (!ctx) return -1;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return getrf_impl<double>(ctx, m, n, a, lda, ipiv, nb, kGemmF64, singular);
+}
+
+lair_status lair_sgetrf_device(lair_context *ctx, int64_t m, int64_t n, float *a, int64_t lda,
+                               int64_t *ipiv, int64_t nb, int64_t *singular) {
+  if (!ctx) return -1;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return getrf_impl<float>(ctx, m, n, a, lda, ipiv, nb, kGemmF32, singular);
+}
+
+lair_status lair_dgetrs_device(lair_context *ctx, int64_t n, int64_t nrhs, const double *lu,
+                               int64_t lda, const int64_t *ipiv, double *b, int64_t ldb,
+                               int32_t trans) {
+  if (!ctx) return -1;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return getrs_impl<double>(ctx, n, nrhs, lu, lda, ipiv, b, ldb, trans);
+}
+
+lair_status lair_sgetrs_device(lair_context *ctx, int64_t n, int64_t nrhs, const float *lu,
+                               int64_t lda, const int64_t *ipiv, float *b, int64_t ldb,
+                               int32_t trans) {
+  if (!ctx) return -1;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return getrs_impl<float>(ctx, n, nrhs, lu, lda, ipiv, b, ldb, trans);
+}
+
+lair_status lair_dgetrf(int64_t m, int64_t n, double *a, int64_t rs, int64_t cs, int64_t *ipiv,
+                        int64_t *singular) {
+  return host_getrf<double>(m, n, a, rs, cs, ipiv, singular);
+}
+lair_status lair_sgetrf(int64_t m, int64_t n, float *a, int64_t rs, int64_t cs, int64_t *ipiv,
+                        int64_t *singular) {
+  return host_getrf<float>(m, n, a, rs, cs, ipiv, singular);
+}
+
+lair_status lair_dgetrs(int64_t n, int64_t nrhs, const double *lu, int64_t lrs, int64_t lcs,
+                        const int64_t *ipiv, double *b, int64_t brs, int64_t bcs, int32_t trans) {
+  return host_getrs<double>(n, nrhs, lu, lrs, lcs, ipiv, b, brs, bcs, trans);
+}
+lair_status lair_sgetrs(int64_t n, int64_t nrhs, const float *lu, int64_t lrs, int64_t lcs,
+                        const int64_t *ipiv, float *b, int64_t brs, int64_t bcs, int32_t trans) {
+  return host_getrs<float>(n, nrhs, lu, lrs, lcs, ipiv, b, brs, bcs, trans);
+}
+
+lair_status lair_dlaswp(int64_t ncols, double *a, int64_t rs, int64_t cs, int64_t begin,
+                        int64_t npiv, const int64_t *ipiv) {
+  if (ncols < 0) return -1;
+  if (npiv < 0) return -6;
+  if (begin < 0) return -5;
+  if (npiv == 0 || begin >= npiv) return LAIR_OK;
+  if (!a) return -2;
+  if (!ipiv) return -7;
+  int64_t rows = npiv;
+  for (int64_t i = begin; i < npiv; ++i) {
+    if (ipiv[i] < 0) return -7;
+    rows = std::max(rows, ipiv[i] + 1);
+  }
+  // The reference swaps the first column pair even when ncols == 0
+  // (laswp.rs:28-38); keep that by treating 0 as 1.
+  const int64_t nc = ncols == 0 ? 1 : ncols;
+  lair_context *ctx = default_ctx();
+  if (!ctx) return LAIR_ERR_DEVICE;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  std::vector<double> h((size_t)(rows * nc));
+  gather(rows, nc, a, rs, cs, h.data(), nc);
+  LAIR_HIP(ensure(ctx->ws_tmp, sizeof(double) * rows * nc));
+  LAIR_HIP(ensure(ctx->ws_ipiv, sizeof(int64_t) * npiv));
+  double *da = static_cast<double *>(ctx->ws_tmp.ptr);
+  int64_t *dp = static_cast<int64_t *>(ctx->ws_ipiv.ptr);
+  LAIR_HIP(hipMemcpyAsync(da, h.data(), sizeof(double) * rows * nc, hipMemcpyHostToDevice, ctx->stream));
+  LAIR_HIP(hipMemcpyAsync(dp, ipiv, sizeof(int64_t) * npiv, hipMemcpyHostToDevice, ctx->stream));
+  hipError_t e = launch_laswp<double>(ctx, da, nc, 0, (int)nc, 0, 0, dp, (int)begin, (int)npiv, false);
+  if (e != hipSuccess) return hip_status(e, "laswp");
+  LAIR_HIP(hipMemcpyAsync(h.data(), da, sizeof(double) * rows * nc, hipMemcpyDeviceToHost, ctx->stream));
+  LAIR_HIP(hipStreamSynchronize(ctx->stream));
+  scatter(rows, nc, h.data(), nc, a, rs, cs);
+  return LAIR_OK;
+}
+
+lair_status lair_dsolve(int64_t m, int64_t n, const double *a, int64_t rs, int64_t cs,
+                        int64_t blen, const double *b, double *x) {
+  if (m < 0 || n < 0 || blen < 0) return -1;
+  if (m != n) {
+    set_error("input matrix is not square");
+    return LAIR_ERR_SHAPE;
+  }
+  if (blen != m) {
+    set_error("The number of elements in `b` must be the same as the number of rows in `a`");
+    return LAIR_ERR_SHAPE;
+  }
+  if (n == 0) return LAIR_OK;
+  std::vector<double> w((size_t)(n * n));
+  gather(n, n, a, rs, cs, w.data(), n);
+  std::vector<int64_t> piv((size_t)n);
+  int64_t sing = -1;
+  lair_status st = lair_dgetrf(n, n, w.data(), n, 1, piv.data(), &sing);
+  if (st != LAIR_OK) return st;
+  if (sing >= 0) {
+    set_error("`a` is a singular matrix");
+    return LAIR_ERR_SINGULAR;
+  }
+  std::memcpy(x, b, sizeof(double) * n);
+  return lair_dgetrs(n, 1, w.data(), n, 1, piv.data(), x, 1, 1, 0);
+}
+
+// ---- device solvers ------------------------------------------------------
+
+static lair_status dgesv_device_locked(lair_context *ctx, int64_t n, int64_t nrhs, const double *a,
+                                       int64_t lda, const double *b, int64_t ldb, double *x,
+                                       int64_t ldx) {
+  const int64_t ld = pad_ld<double>(n);
+  LAIR_HIP(ensure(ctx->ws_lu64, sizeof(double) * n * ld));
+  LAIR_HIP(ensure(ctx->ws_ipiv, sizeof(int64_t) * n));
+  double *lu = static_cast<double *>(ctx->ws_lu64.ptr);
+  int64_t *piv = static_cast<int64_t *>(ctx->ws_ipiv.ptr);
+  hipError_t e = launch_copy_d2d(ctx, (int)n, (int)n, a, lda, lu, ld);
+  if (e != hipSuccess) return hip_status(e, "copy");
+  int64_t sing = -1;
+  lair_status st = getrf_impl<double>(ctx, n, n, lu, ld, piv, 0, kGemmF64, &sing);
+  if (st != LAIR_OK) return st;
+  if (sing >= 0) {
+    set_error("`a` is a singular matrix");
+    return LAIR_ERR_SINGULAR;
+  }
+  e = launch_copy_d2d(ctx, (int)n, (int)nrhs, b, ldb, x, ldx);
+  if (e != hipSuccess) return hip_status(e, "copy");
+  return getrs_impl<double>(ctx, n, nrhs, lu, ld, piv, x, ldx, 0);
+}
+
+lair_status lair_dgesv_device(lair_context *ctx, int64_t n, int64_t nrhs, const double *a,
+                              int64_t lda, const double *b, int64_t ldb, double *x, int64_t ldx) {
+  if (!ctx) return -1;
+  if (n < 0) return -2;
+  if (nrhs < 0) return -3;
+  if (n == 0 || nrhs == 0) return LAIR_OK;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  lair_status st = dgesv_device_locked(ctx, n, nrhs, a, lda, b, ldb, x, ldx);
+  if (st != LAIR_OK) return st;
+  LAIR_HIP(hipStreamSynchronize(ctx->stream));
+  return LAIR_OK;
+}
+
+static lair_status berr_locked(lair_context *ctx, int n, int nrhs, const double *a, int64_t lda,
+                               const double *b, int64_t ldb, const double *x, int64_t ldx,
+                               double *berr, bool norm_a_ready) {
+  LAIR_HIP(ensure(ctx->ws_r, sizeof(double) * n * nrhs));
+  double *r = static_cast<double *>(ctx->ws_r.ptr);
+  unsigned long long *norms = reinterpret_cast<unsigned long long *>(ctx->norms);
+  hipError_t e;
+  if (!norm_a_ready) {
+    e = launch_fill_f64(ctx, ctx->norms, 1, 0.0);
+    if (e != hipSuccess) return hip_status(e, "fill");
+    e = launch_rows_norm(ctx, n, n, a, lda, norms);
+    if (e != hipSuccess) return hip_status(e, "norm");
+  }
+  e = launch_residual(ctx, n, nrhs, a, lda, x, ldx, b, ldb, r, nrhs);
+  if (e != hipSuccess) return hip_status(e, "residual");
+  return backward_error_from(ctx, n, nrhs, r, nrhs, x, ldx, norms, berr);
+}
+
+lair_status lair_dbackward_error(lair_context *ctx, int64_t n, int64_t nrhs, const double *a,
+                                 int64_t lda, const double *b, int64_t ldb, const double *x,
+                                 int64_t ldx, double *berr) {
+  if (!ctx) return -1;
+  if (!berr) return -10;
+  if (n <= 0 || nrhs <= 0) {
+    *berr = 0.0;
+    return LAIR_OK;
+  }
+  if (nrhs > 2000) return -3;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return berr_locked(ctx, (int)n, (int)nrhs, a, lda, b, ldb, x, ldx, berr, false);
+}
+
+lair_status lair_dgesv_mixed_device(lair_context *ctx, int64_t n64, int64_t nrhs64, const double *a,
+                                    int64_t lda, const double *b, int64_t ldb, double *x,
+                                    int64_t ldx, int32_t prec, int32_t max_iter, double tol,
+                                    lair_ir_info *info) {
+  if (!ctx) return -1;
+  if (n64 < 0) return -2;
+  if (nrhs64 < 0 || nrhs64 > 2000) return -3;
+  if (prec < LAIR_PREC_FP64 || prec > LAIR_PREC_BF16) return -11;
+  if (max_iter < 0 || max_iter > 30) return -12;
+  lair_ir_info local;
+  if (!info) info = &local;
+  std::memset(info, 0, sizeof(*info));
+  info->precision = prec;
+  if (n64 == 0 || nrhs64 == 0) {
+    info->converged = 1;
+    return LAIR_OK;
+  }
+  const int n = (int)n64, nrhs = (int)nrhs64;
+  if (tol <= 0.0) tol = (double)n * DBL_EPSILON;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  hipError_t e;
+  unsigned long long *norms = reinterpret_cast<unsigned long long *>(ctx->norms);
+  if (prec == LAIR_PREC_FP64) {
+    lair_status st = dgesv_device_locked(ctx, n, nrhs, a, lda, b, ldb, x, ldx);
+    if (st != LAIR_OK) return st;
+    double be;
+    st = berr_locked(ctx, n, nrhs, a, lda, b, ldb, x, ldx, &be, false);
+    if (st != LAIR_OK) return st;
+    info->backward_error = be;
+    info->history[0] = be;
+    info->converged = be <= tol;
+    return LAIR_OK;
+  }
+  const int mode = prec == LAIR_PREC_FP32 ? kGemmF32 : (prec == LAIR_PREC_BF16X6 ? kGemmBF16x6 : kGemmBF16);
+  const int64_t ld32 = pad_ld<float>(n);
+  LAIR_HIP(ensure(ctx->ws_a32, sizeof(float) * n * ld32));
+  LAIR_HIP(ensure(ctx->ws_ipiv, sizeof(int64_t) * n));
+  LAIR_HIP(ensure(ctx->ws_x32, sizeof(float) * n * nrhs));
+  LAIR_HIP(ensure(ctx->ws_r, sizeof(double) * n * nrhs));
+  float *a32 = static_cast<float *>(ctx->ws_a32.ptr);
+  int64_t *piv = static_cast<int64_t *>(ctx->ws_ipiv.ptr);
+  float *d32 = static_cast<float *>(ctx->ws_x32.ptr);
+  double *r = static_cast<double *>(ctx->ws_r.ptr);
+  e = launch_fill_f64(ctx, ctx->norms, 1, 0.0);
+  if (e != hipSuccess) return hip_status(e, "fill");
+  e = launch_convert_d2s_norm(ctx, n, n, a, lda, a32, ld32, norms);
+  if (e != hipSuccess) return hip_status(e, "convert");
+  lair_status st = getrf_impl<float>(ctx, n, n, a32, ld32, piv, 0, mode, nullptr);
+  if (st != LAIR_OK) return st;
+  // Initial solve in the factorization precision.
+  e = launch_copy_d2s(ctx, n, nrhs, b, ldb, d32, nrhs);
+  if (e != hipSuccess) return hip_status(e, "copy");
+  st = getrs_impl<float>(ctx, n, nrhs, a32, ld32, piv, d32, nrhs, 0);
+  if (st != LAIR_OK) return st;
+  e = launch_copy_s2d(ctx, n, nrhs, d32, nrhs, x, ldx);
+  if (e != hipSuccess) return hip_status(e, "copy");
+  bool converged = false;
+  int it = 0;
+  double be = INFINITY;
+  for (;;) {
+    e = launch_residual(ctx, n, nrhs, a, lda, x, ldx, b, ldb, r, nrhs);
+    if (e != hipSuccess) return hip_status(e, "residual");
+    st = backward_error_from(ctx, n, nrhs, r, nrhs, x, ldx, norms, &be);
+    if (st != LAIR_OK) return st;
+    if (it < 32) info->history[it] = be;
+    if (be <= tol) {
+      converged = true;
+      break;
+    }
+    if (it >= max_iter || !std::isfinite(be)) break;
+    // Correction d = LU^-1 r in the factorization precision, x += d in FP64.
+    e = launch_copy_d2s(ctx, n, nrhs, r, nrhs, d32, nrhs);
+    if (e != hipSuccess) return hip_status(e, "copy");
+    st = getrs_impl<float>(ctx, n, nrhs, a32, ld32, piv, d32, nrhs, 0);
+    if (st != LAIR_OK) return st;
+    e = launch_axpy_s2d(ctx, n, nrhs, d32, nrhs, x, ldx);
+    if (e != hipSuccess) return hip_status(e, "axpy");
+    ++it;
+  }
+  info->iterations = it;
+  info->backward_error = be;
+  info->converged = converged ? 1 : 0;
+  if (converged) return LAIR_OK;
+  // Fallback: FP64 factorization.
+  st = dgesv_device_locked(ctx, n, nrhs, a, lda, b, ldb, x, ldx);
+  if (st != LAIR_OK) return st;
+  st = berr_locked(ctx, n, nrhs, a, lda, b, ldb, x, ldx, &be, true);
+  if (st != LAIR_OK) return st;
+  info->fell_back = 1;
+  info->precision = LAIR_PREC_FP64;
+  info->backward_error = be;
+  return LAIR_ERR_NOCONV;
+}
+
+// ---- instrumentation -----------------------------------------------------
+
+lair_status lair_profile_enable(lair_context *ctx, int32_t on) {
+  if (!ctx) return -1;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->profiling = on != 0;
+  return LAIR_OK;
+}
+
+lair_status lair_profile_read(lair_context *ctx, int32_t kclass, double *ms, int64_t *launches,
+                              double *flops) {
+  if (!ctx) return -1;
+  if (kclass < 0 || kclass >= LAIR_KCLASS_COUNT) return -2;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  prof_collect(ctx);
+  if (ms) *ms = ctx->prof_ms[kclass];
+  if (launches) *launches = ctx->prof_n[kclass];
+  if (flops) *flops = ctx->prof_flops[kclass];
+  return LAIR_OK;
+}
+
+lair_status lair_profile_reset(lair_context *ctx) {
+  if (!ctx) return -1;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  prof_collect(ctx);
+  for (int i = 0; i < LAIR_KCLASS_COUNT; ++i) {
+    ctx->prof_ms[i] = 0;
+    ctx->prof_n[i] = 0;
+    ctx->prof_flops[i] = 0;
+  }
+  return LAIR_OK;
+}
+
+}  // extern "C"
