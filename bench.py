@@ -1,0 +1,251 @@
+#!/usr/bin/env python3
+"""Headline benchmark: n=16384 FP64 solve time, mixed-precision LU + FP64
+iterative refinement against the FP64 LU, at equal backward error.
+
+Workload (one "step" = one full solve A X = B from device-resident FP64 data):
+  A = U diag(s) V^T, U and V Haar orthogonal (QR of a Gaussian matrix with the
+  sign fix), s log-spaced from 1 to 1e6 (cond_2(A) = 1e6), n = 16384; B n x nrhs
+  i.i.d. N(0,1).  Inputs are generated on the GPU and are not timed.
+
+Printed JSON line: value = milliseconds per mixed-precision solve (lower is
+better) with the FP64 solve time, the speedup and both backward errors in
+`config`; `roofline` for the dominant kernel (trailing-matrix update) from HIP
+events recorded on the library's stream during the timed steps; `cpu_baseline`
+from the CPU oracle on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 16384] [--nrhs 1]
+"""
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "n=16384 solve time: BF16/FP32-MFMA LU+IR vs FP64 LU at equal backward error"
+PEAK = {"hbm_gbs": 8000.0, "f64": 78.6, "f32": 157.3, "bf16": 2500.0}
+PREC_NAMES = {"fp32": 1, "bf16x6": 2}
+EPS = float(np.finfo(np.float64).eps)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def gen_lib():
+    path = os.path.join(ROOT, "lair_amd", "lib", "liblair_gen.so")
+    lib = ctypes.CDLL(path)
+    lib.lair_gen_gaussian.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                      ctypes.c_uint64]
+    lib.lair_gen_gaussian.restype = ctypes.c_int
+    lib.lair_gen_kappa.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_double,
+                                   ctypes.c_uint64]
+    lib.lair_gen_kappa.restype = ctypes.c_int
+    return lib
+
+
+def prof_read(ctx, kname):
+    from lair_amd import _capi
+    ms, cnt, fl, by = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+    ctx.lib.lair_profile_read(ctx.h, _capi.KCLASS[kname], ctypes.byref(ms), ctypes.byref(cnt),
+                              ctypes.byref(fl), ctypes.byref(by))
+    return ms.value, cnt.value, fl.value, by.value
+
+
+def cpu_baseline(n_target, n_sample):
+    """The oracle's restatement of getrf_row_major + getrs (single thread) on an
+    n_sample Gaussian system, scaled to n_target by the n^3 flop ratio."""
+    from lair_amd import inputs
+    from oracle import oracle as O
+    a = inputs.gaussian_matrix(99, n_sample, n_sample)
+    b = inputs.gaussian_matrix(100, n_sample, 1)
+    t = time.perf_counter()
+    piv, _ = O.getrf(a, nthreads=1)
+    O.getrs(a, piv, b)
+    dt = time.perf_counter() - t
+    scale = (n_target / n_sample) ** 3
+    return {"value": round(dt * scale * 1e3, 1), "unit": "ms",
+            "cores": 1, "kind": "port",
+            "sample": f"oracle getrf_row_major+getrs on one core, n={n_sample} N(0,1), "
+                      f"{dt:.2f} s measured, scaled by (n/{n_sample})^3 to n={n_target}",
+            "cpu": platform.processor() or platform.machine()}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=16384)
+    ap.add_argument("--nrhs", type=int, default=1)
+    ap.add_argument("--kappa", type=float, default=1e6)
+    ap.add_argument("--precisions", default="bf16x6,fp32")
+    ap.add_argument("--cpu-sample-n", type=int, default=2048)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the n=16384 LU "
+              "fits one MI355X); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import lair_amd as L
+    from lair_amd import _capi
+
+    n, nrhs, K, W = args.n, args.nrhs, args.steps, args.warmup
+    ld = n + 64 if (n % 4096 == 0) else ((n + 63) // 64) * 64
+    ctx = L.Context()
+    lib = ctx.lib
+    gl = gen_lib()
+    da = L.DeviceArray(n, n, np.float64, ld=ld)
+    db = L.DeviceArray(n, nrhs)
+    dx = L.DeviceArray(n, nrhs)
+    t = time.perf_counter()
+    rc = gl.lair_gen_kappa(da.ptr, n, ld, args.kappa, 16384)
+    rc |= gl.lair_gen_gaussian(db.ptr, n, nrhs, nrhs, 16385)
+    if rc:
+        raise RuntimeError(f"input generation failed ({rc})")
+    log(f"generated A (n={n}, kappa={args.kappa:g}) and B ({nrhs} rhs) in {time.perf_counter() - t:.1f} s")
+
+    # Host-to-device transfer of A, reported separately (never the headline).
+    host = np.empty((n, ld), np.float64)
+    lib.lair_memcpy_d2h(ctypes.c_void_p(host.ctypes.data), da.ptr, host.nbytes)
+    t = time.perf_counter()
+    lib.lair_memcpy_h2d(da.ptr, ctypes.c_void_p(host.ctypes.data), host.nbytes)
+    h2d_ms = (time.perf_counter() - t) * 1e3
+    del host
+
+    def berr():
+        v = ctypes.c_double()
+        L.check(lib.lair_dbackward_error(ctx.h, n, nrhs, da.ptr, ld, db.ptr, nrhs, dx.ptr, nrhs,
+                                         ctypes.byref(v)), "berr")
+        return v.value
+
+    # FP64 LU + getrs.
+    def fp64_step():
+        L.check(lib.lair_dgesv_device(ctx.h, n, nrhs, da.ptr, ld, db.ptr, nrhs, dx.ptr, nrhs), "dgesv")
+
+    for _ in range(W):
+        fp64_step()
+    ctx.synchronize()
+    t = time.perf_counter()
+    for _ in range(K):
+        fp64_step()
+    ctx.synchronize()
+    fp64_ms = (time.perf_counter() - t) * 1e3 / K
+    fp64_berr = berr()
+    log(f"fp64 LU: {fp64_ms:.2f} ms/solve, backward error {fp64_berr:.2e}")
+
+    results = {}
+    for name in [p for p in args.precisions.split(",") if p]:
+        prec = PREC_NAMES[name]
+        info = _capi.IrInfo()
+
+        def step():
+            st = lib.lair_dgesv_mixed_device(ctx.h, n, nrhs, da.ptr, ld, db.ptr, nrhs, dx.ptr, nrhs, prec,
+                                             20, 0.0, ctypes.byref(info))
+            if st not in (0, _capi.LAIR_ERR_NOCONV):
+                L.check(st, "dgesv_mixed")
+            return st
+
+        for _ in range(W):
+            step()
+        lib.lair_profile_reset(ctx.h)
+        # Events around the trailing-update kernels only (the dominant class).
+        lib.lair_profile_enable(ctx.h, 1 << _capi.KCLASS["gemm"])
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(K):
+            st = step()
+        ctx.synchronize()
+        ms = (time.perf_counter() - t) * 1e3 / K
+        lib.lair_profile_enable(ctx.h, 0)
+        g_ms, g_cnt, g_fl, g_by = prof_read(ctx, "gemm")
+        be = berr()
+        results[name] = {"ms": ms, "status": st, "iterations": info.iterations,
+                         "converged": info.converged, "fell_back": info.fell_back,
+                         "backward_error": be, "gemm": (g_ms / K, g_cnt // K, g_fl / K, g_by / K)}
+        log(f"{name}: {ms:.2f} ms/solve, {info.iterations} IR steps, backward error {be:.2e}, "
+            f"fell_back={info.fell_back}, trailing gemm {g_ms / K:.2f} ms over {g_cnt // K} launches")
+
+    # Per-class time breakdown of one extra profiled solve (headline precision).
+    ok = {k: v for k, v in results.items() if v["converged"] and not v["fell_back"]}
+    head = min(ok, key=lambda k: ok[k]["ms"]) if ok else min(results, key=lambda k: results[k]["ms"])
+    hr = results[head]
+    lib.lair_profile_reset(ctx.h)
+    lib.lair_profile_enable(ctx.h, 0xFF)
+    info = _capi.IrInfo()
+    lib.lair_dgesv_mixed_device(ctx.h, n, nrhs, da.ptr, ld, db.ptr, nrhs, dx.ptr, nrhs, PREC_NAMES[head], 20,
+                                0.0, ctypes.byref(info))
+    lib.lair_profile_enable(ctx.h, 0)
+    breakdown = {k: round(prof_read(ctx, k)[0], 3) for k in _capi.KCLASS}
+
+    # Roofline of the trailing update: the lower of the MFMA and HBM ceilings.
+    g_ms, g_cnt, g_fl, g_by = hr["gemm"]
+    mfma_peak = PEAK["bf16"] if head.startswith("bf16") else PEAK["f32"]
+    products = 6 if head == "bf16x6" else 1
+    t_mfma = g_fl * products / (mfma_peak * 1e12)
+    t_hbm = g_by / (PEAK["hbm_gbs"] * 1e9)
+    if t_hbm >= t_mfma:
+        achieved = g_by / (g_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK["hbm_gbs"], "unit": "GB/s",
+                "frac": round(achieved / PEAK["hbm_gbs"], 4)}
+    else:
+        achieved = g_fl / (g_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": mfma_peak, "unit": "TFLOP/s",
+                "frac": round(achieved / mfma_peak, 4)}
+    roof.update({"traffic": None, "kernel": f"trailing update ({head})",
+                 "launches_per_step": g_cnt, "algorithmic_flops_per_step": g_fl,
+                 "algorithmic_bytes_per_step": g_by, "ms_per_step": round(g_ms, 3)})
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(n, args.cpu_sample_n)
+        except Exception as e:  # the oracle build is optional on a box without gcc
+            log(f"cpu baseline skipped: {e}")
+
+    line = {
+        "metric": METRIC,
+        "value": round(hr["ms"], 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(hr["ms"], 3),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "bf16x6-mfma+f64-ir" if head == "bf16x6" else "f32-mfma+f64-ir",
+        "data": "synthetic",
+        "config": {
+            "workload": f"dgesv n={n} nrhs={nrhs} kappa={args.kappa:g} (Haar U,V; log-spaced s)",
+            "n": n, "nrhs": nrhs, "kappa": args.kappa, "precision": head,
+            "fp64_ms": round(fp64_ms, 3),
+            "speedup_vs_fp64": round(fp64_ms / hr["ms"], 3),
+            "backward_error": hr["backward_error"],
+            "fp64_backward_error": fp64_berr,
+            "tolerance": n * EPS,
+            "ir_iterations": hr["iterations"],
+            "variants": {k: {"ms": round(v["ms"], 3), "iterations": v["iterations"],
+                             "backward_error": v["backward_error"], "fell_back": v["fell_back"]}
+                         for k, v in results.items()},
+            "h2d_ms_A": round(h2d_ms, 2),
+            "breakdown_ms": breakdown,
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    for d in (da, db, dx):
+        d.free()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

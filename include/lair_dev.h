@@ -166,17 +166,21 @@ lair_status lair_dbackward_error(lair_context *ctx, int64_t n, int64_t nrhs, con
 #define LAIR_KCLASS_PANEL 0
 #define LAIR_KCLASS_LASWP 1
 #define LAIR_KCLASS_TRSM 2
-#define LAIR_KCLASS_GEMM 3
+#define LAIR_KCLASS_GEMM 3 /* trailing-matrix update A22 -= L21 U12 */
 #define LAIR_KCLASS_GETRS 4
 #define LAIR_KCLASS_RESIDUAL 5
 #define LAIR_KCLASS_CONVERT 6
-#define LAIR_KCLASS_COUNT 7
+#define LAIR_KCLASS_GEMM_PANEL 7 /* updates inside a panel (narrow K) */
+#define LAIR_KCLASS_COUNT 8
 
-lair_status lair_profile_enable(lair_context *ctx, int32_t on);
-/* Accumulated milliseconds, launch count, and algorithmic flops for a class
- * since the last reset (synchronizes the stream). */
+/* class_mask: bit k enables class k; 0 disables; LAIR_PROFILE_ALL enables all. */
+#define LAIR_PROFILE_ALL 0xFF
+lair_status lair_profile_enable(lair_context *ctx, int32_t class_mask);
+/* Accumulated milliseconds, launch count, algorithmic flops and algorithmic
+ * bytes (minimum HBM traffic of the launches' operands) for a class since the
+ * last reset (synchronizes the stream). */
 lair_status lair_profile_read(lair_context *ctx, int32_t kclass, double *ms, int64_t *launches,
-                              double *flops);
+                              double *flops, double *bytes);
 lair_status lair_profile_reset(lair_context *ctx);
 
 #ifdef __cplusplus

@@ -30,7 +30,8 @@ PREC_FP32 = 1
 PREC_BF16X6 = 2
 PREC_BF16 = 3
 
-KCLASS = {"panel": 0, "laswp": 1, "trsm": 2, "gemm": 3, "getrs": 4, "residual": 5, "convert": 6}
+KCLASS = {"panel": 0, "laswp": 1, "trsm": 2, "gemm": 3, "getrs": 4, "residual": 5, "convert": 6,
+          "gemm_panel": 7}
 
 
 class IrInfo(ctypes.Structure):
@@ -71,7 +72,7 @@ SIGNATURES = [
     ("lair_dgesv_device", i32, [vp, i64, i64, vp, i64, vp, i64, vp, i64]),
     ("lair_dbackward_error", i32, [vp, i64, i64, vp, i64, vp, i64, vp, i64, PD]),
     ("lair_profile_enable", i32, [vp, i32]),
-    ("lair_profile_read", i32, [vp, i32, PD, P64, PD]),
+    ("lair_profile_read", i32, [vp, i32, PD, P64, PD, PD]),
     ("lair_profile_reset", i32, [vp]),
 ]
 

@@ -11,9 +11,13 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = ["kernels.hip", "driver.hip"]
+SOURCES = ["kernels.hip", "solve_kernels.hip", "gemm_kernels.hip", "driver.hip"]
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "liblair_dev.so")
+# Input generator for the benchmark and large tests (links rocSOLVER/rocBLAS;
+# never loaded by the solver library itself).
+GEN_SOURCES = ["gen_inputs.hip"]
+GEN_LIB = os.path.join(LIB_DIR, "liblair_gen.so")
 ARCH = os.environ.get("LAIR_OFFLOAD_ARCH", "gfx950")
 
 
@@ -31,17 +35,27 @@ def command(out=LIB, extra=()):
             "-Wno-unused-result", *extra, "-o", out, *srcs]
 
 
-def build(verbose=False):
-    os.makedirs(LIB_DIR, exist_ok=True)
-    tmp = LIB + ".tmp"
-    cmd = command(tmp)
+def gen_command(out=GEN_LIB):
+    srcs = [os.path.join(HERE, "csrc", s) for s in GEN_SOURCES]
+    return [hipcc(), "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-shared",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-o", out, *srcs, "-lrocsolver", "-lrocblas"]
+
+
+def _run(cmd, tmp, final, verbose):
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building liblair_dev.so")
+        raise RuntimeError(f"hipcc failed building {os.path.basename(final)}")
     if verbose:
         sys.stderr.write(res.stderr)
-    os.replace(tmp, LIB)
+    os.replace(tmp, final)
+
+
+def build(verbose=False):
+    os.makedirs(LIB_DIR, exist_ok=True)
+    _run(command(LIB + ".tmp"), LIB + ".tmp", LIB, verbose)
+    _run(gen_command(GEN_LIB + ".tmp"), GEN_LIB + ".tmp", GEN_LIB, verbose)
     return LIB
 
 

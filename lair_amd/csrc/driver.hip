@@ -48,8 +48,8 @@ hipError_t ensure(Buffer &b, size_t bytes) {
   return hipSuccess;
 }
 
-int prof_begin(lair_context *ctx) {
-  if (!ctx->profiling) return -1;
+int prof_begin(lair_context *ctx, int kclass) {
+  if (!(ctx->prof_mask & (1 << kclass))) return -1;
   if (ctx->event_next + 2 > ctx->event_pool.size()) {
     for (int i = 0; i < 64; ++i) {
       hipEvent_t e;
@@ -63,10 +63,10 @@ int prof_begin(lair_context *ctx) {
   return slot;
 }
 
-void prof_end(lair_context *ctx, int slot, int kclass, double flops) {
+void prof_end(lair_context *ctx, int slot, int kclass, double flops, double bytes) {
   if (slot < 0) return;
   hipEventRecord(ctx->event_pool[slot + 1], ctx->stream);
-  ctx->prof.push_back({kclass, flops, ctx->event_pool[slot], ctx->event_pool[slot + 1]});
+  ctx->prof.push_back({kclass, flops, bytes, ctx->event_pool[slot], ctx->event_pool[slot + 1]});
 }
 
 static void prof_collect(lair_context *ctx) {
@@ -77,6 +77,7 @@ static void prof_collect(lair_context *ctx) {
     if (hipEventElapsedTime(&ms, r.start, r.stop) == hipSuccess) ctx->prof_ms[r.kclass] += ms;
     ctx->prof_n[r.kclass] += 1;
     ctx->prof_flops[r.kclass] += r.flops;
+    ctx->prof_bytes[r.kclass] += r.bytes;
   }
   ctx->prof.clear();
   ctx->event_next = 0;
@@ -111,11 +112,13 @@ static lair_status hip_status(hipError_t e, const char *what) {
 // kLeafW columns with exact partial pivoting; the leaf's interchanges are
 // applied to the rest of the panel, the leaf's U rows solved and the rest of
 // the panel updated before the next leaf.  After the panel, its interchanges
-// are applied to the columns left and right of it, U12 = L11^-1 A12 and
+// are applied to the columns left and right of it (one gather, which also
+// updates the row permutation perm when given), U12 = L11^-1 A12 and
 // A22 -= L21 U12.
 template <typename T>
 static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a, int64_t lda,
-                              int64_t *ipiv, int64_t nb, int mode, int64_t *singular) {
+                              int64_t *ipiv, int64_t nb, int mode, int64_t *singular,
+                              int *perm = nullptr) {
   if (m64 < 0) return -2;
   if (n64 < 0) return -3;
   if (lda < std::max<int64_t>(n64, 1)) return -5;
@@ -131,11 +134,18 @@ static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a,
   nb = (nb / kLeafW) * kLeafW;
   hipError_t e = launch_fill_i64(ctx, ctx->sing, 1, -1);
   if (e != hipSuccess) return hip_status(e, "fill");
-  // Keep sequence numbers far from wrap-around.
-  if (ctx->seq > 0x70000000u) {
-    e = launch_fill_u32(ctx, ctx->flags, kMaxPanelWG, 0u);
+  if (perm) {
+    e = launch_iota(ctx, perm, m);
+    if (e != hipSuccess) return hip_status(e, "iota");
+  }
+  // Keep exchange tags far from wrap-around (a reused tag could match a stale
+  // record): clear the records and restart the tags.
+  if (ctx->seq > 0xF0000000u) {
+    e = launch_fill_u32(ctx, reinterpret_cast<uint32_t *>(ctx->rec), 4 * kMaxPanelWG * 64, 0u);
     if (e != hipSuccess) return hip_status(e, "fill");
-    ctx->seq = 0;
+    e = launch_fill_u32(ctx, reinterpret_cast<uint32_t *>(ctx->drec), 4 * 64, 0u);
+    if (e != hipSuccess) return hip_status(e, "fill");
+    ctx->seq = 1;
   }
   for (int k = 0; k < kmax; k += (int)nb) {
     const int kb = std::min<int>((int)nb, kmax - k);
@@ -144,26 +154,26 @@ static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a,
       e = launch_panel<T>(ctx, a, lda, c, m, c, w, ipiv);
       if (e != hipSuccess) return hip_status(e, "panel");
       if (kb > w) {
-        e = launch_laswp<T>(ctx, a, lda, k, k + kb, c, c + w, ipiv, c, c + w, false);
-        if (e != hipSuccess) return hip_status(e, "laswp");
+        e = launch_rowswap<T>(ctx, a, lda, k, k + kb, c, c + w, ipiv, c, c + w, nullptr);
+        if (e != hipSuccess) return hip_status(e, "rowswap");
       }
       const int rest = k + kb - (c + w);
       if (rest > 0) {
-        e = launch_trsm<T>(ctx, true, false, true, w, rest, a + (int64_t)c * lda + c, lda,
-                           a + (int64_t)c * lda + c + w, lda);
+        e = launch_trsm_llu<T>(ctx, w, rest, a + (int64_t)c * lda + c, lda,
+                               a + (int64_t)c * lda + c + w, lda);
         if (e != hipSuccess) return hip_status(e, "trsm");
         e = launch_gemm_minus<T>(ctx, mode, m - (c + w), rest, w, a + (int64_t)(c + w) * lda + c,
                                  lda, false, a + (int64_t)c * lda + c + w, lda,
-                                 a + (int64_t)(c + w) * lda + c + w, lda);
+                                 a + (int64_t)(c + w) * lda + c + w, lda, LAIR_KCLASS_GEMM_PANEL);
         if (e != hipSuccess) return hip_status(e, "gemm");
       }
     }
-    e = launch_laswp<T>(ctx, a, lda, 0, n, k, k + kb, ipiv, k, k + kb, false);
-    if (e != hipSuccess) return hip_status(e, "laswp");
+    e = launch_rowswap<T>(ctx, a, lda, 0, n, k, k + kb, ipiv, k, k + kb, perm);
+    if (e != hipSuccess) return hip_status(e, "rowswap");
     const int right = n - (k + kb);
     if (right > 0) {
-      e = launch_trsm<T>(ctx, true, false, true, kb, right, a + (int64_t)k * lda + k, lda,
-                         a + (int64_t)k * lda + k + kb, lda);
+      e = launch_trsm_llu<T>(ctx, kb, right, a + (int64_t)k * lda + k, lda,
+                             a + (int64_t)k * lda + k + kb, lda);
       if (e != hipSuccess) return hip_status(e, "trsm");
       if (m - (k + kb) > 0) {
         e = launch_gemm_minus<T>(ctx, mode, m - (k + kb), right, kb, a + (int64_t)(k + kb) * lda + k,
@@ -186,9 +196,79 @@ static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a,
   return LAIR_OK;
 }
 
-// Blocked getrs (getrs.rs:12-39 generalized to nrhs columns and to A^T):
-//   A X = B   : forward laswp, L (unit) forward blocks, U backward blocks;
-//   A^T X = B : U^T forward blocks, L^T (unit) backward blocks, reverse laswp.
+// Solve-side data derived from a factorization: the row permutation and the
+// inverses of the diagonal blocks of L and U.
+template <typename T>
+struct SolveAux {
+  const int *perm = nullptr;
+  const T *linv = nullptr;
+  const T *uinv = nullptr;
+};
+
+// Builds SolveAux in the context's workspace.  perm is taken from perm_in
+// when the factorization produced it, otherwise composed from ipiv.
+template <typename T>
+static lair_status make_aux(lair_context *ctx, int n, const T *lu, int64_t lda, const int64_t *ipiv,
+                            const int *perm_in, SolveAux<T> *aux) {
+  constexpr int BS = TrsvBlock<T>::BS;
+  const int nblk = (n + BS - 1) / BS;
+  LAIR_HIP(ensure(ctx->ws_linv, sizeof(T) * (size_t)nblk * BS * BS));
+  LAIR_HIP(ensure(ctx->ws_uinv, sizeof(T) * (size_t)nblk * BS * BS));
+  T *linv = static_cast<T *>(ctx->ws_linv.ptr);
+  T *uinv = static_cast<T *>(ctx->ws_uinv.ptr);
+  hipError_t e;
+  if (!perm_in) {
+    LAIR_HIP(ensure(ctx->ws_perm, sizeof(int) * (size_t)n));
+    int *perm = static_cast<int *>(ctx->ws_perm.ptr);
+    e = launch_perm_from_ipiv(ctx, ipiv, n, perm);
+    if (e != hipSuccess) return hip_status(e, "perm");
+    perm_in = perm;
+  }
+  e = launch_trtri_diag<T>(ctx, lu, lda, n, linv, uinv);
+  if (e != hipSuccess) return hip_status(e, "trtri");
+  aux->perm = perm_in;
+  aux->linv = linv;
+  aux->uinv = uinv;
+  return LAIR_OK;
+}
+
+// Dataflow getrs: A X = B as gather (P b), L forward, U backward;
+// A^T X = B as U^T forward, L^T backward, scatter (P^T).
+template <typename T>
+static lair_status getrs_aux(lair_context *ctx, int n, int nrhs, const T *lu, int64_t lda,
+                             const SolveAux<T> &aux, T *b, int64_t ldb, int trans) {
+  LAIR_HIP(ensure(ctx->ws_tmp2, sizeof(T) * (size_t)n * nrhs));
+  T *t = static_cast<T *>(ctx->ws_tmp2.ptr);
+  hipError_t e;
+  if (!trans) {
+    e = launch_permute_rows<T, T>(ctx, n, nrhs, aux.perm, b, ldb, t, nrhs, false);
+    if (e != hipSuccess) return hip_status(e, "permute");
+    e = launch_trsv_dataflow<T>(ctx, lu, lda, aux.linv, false, t, t, nrhs, n, nrhs, true, false);
+    if (e != hipSuccess) return hip_status(e, "trsv");
+    if (ldb != nrhs) {
+      e = launch_trsv_dataflow<T>(ctx, lu, lda, aux.uinv, false, t, t, nrhs, n, nrhs, false, false);
+      if (e != hipSuccess) return hip_status(e, "trsv");
+      e = launch_permute_rows<T, T>(ctx, n, nrhs, nullptr, t, nrhs, b, ldb, false);
+    } else {
+      e = launch_trsv_dataflow<T>(ctx, lu, lda, aux.uinv, false, t, b, ldb, n, nrhs, false, false);
+    }
+    if (e != hipSuccess) return hip_status(e, "trsv");
+  } else {
+    e = launch_permute_rows<T, T>(ctx, n, nrhs, nullptr, b, ldb, t, nrhs, false);
+    if (e != hipSuccess) return hip_status(e, "copy");
+    e = launch_trsv_dataflow<T>(ctx, lu, lda, aux.uinv, true, t, t, nrhs, n, nrhs, true, true);
+    if (e != hipSuccess) return hip_status(e, "trsv");
+    e = launch_trsv_dataflow<T>(ctx, lu, lda, aux.linv, true, t, t, nrhs, n, nrhs, false, true);
+    if (e != hipSuccess) return hip_status(e, "trsv");
+    e = launch_permute_rows<T, T>(ctx, n, nrhs, aux.perm, t, nrhs, b, ldb, true);
+    if (e != hipSuccess) return hip_status(e, "scatter");
+  }
+  return LAIR_OK;
+}
+
+// getrs (getrs.rs:12-39 generalized to nrhs columns and to A^T).  Systems up
+// to kExactOrderMax use the reference's exact operation order; larger ones
+// the dataflow solve.
 template <typename T>
 static lair_status getrs_impl(lair_context *ctx, int64_t n64, int64_t nrhs64, const T *lu,
                               int64_t lda, const int64_t *ipiv, T *b, int64_t ldb, int trans) {
@@ -196,71 +276,14 @@ static lair_status getrs_impl(lair_context *ctx, int64_t n64, int64_t nrhs64, co
   if (nrhs64 < 0) return -3;
   if (n64 == 0 || nrhs64 == 0) return LAIR_OK;
   const int n = (int)n64, nrhs = (int)nrhs64;
-  const int bs = 128;
-  hipError_t e;
-  const int mode = std::is_same<T, double>::value ? kGemmF64 : kGemmF32;
   if (!trans && n <= kExactOrderMax) {
-    e = launch_getrs_exact<T>(ctx, n, nrhs, lu, lda, ipiv, b, ldb);
+    hipError_t e = launch_getrs_exact<T>(ctx, n, nrhs, lu, lda, ipiv, b, ldb);
     return hip_status(e, "getrs");
   }
-  if (!trans) {
-    e = launch_laswp<T>(ctx, b, ldb, 0, nrhs, 0, 0, ipiv, 0, n, false);
-    if (e != hipSuccess) return hip_status(e, "laswp");
-    for (int k = 0; k < n; k += bs) {
-      const int kb = std::min(bs, n - k);
-      e = launch_trsm<T>(ctx, true, false, true, kb, nrhs, lu + (int64_t)k * lda + k, lda,
-                         b + (int64_t)k * ldb, ldb);
-      if (e != hipSuccess) return hip_status(e, "trsm");
-      if (k + kb < n) {
-        e = launch_gemm_minus<T>(ctx, mode, n - k - kb, nrhs, kb, lu + (int64_t)(k + kb) * lda + k,
-                                 lda, false, b + (int64_t)k * ldb, ldb, b + (int64_t)(k + kb) * ldb,
-                                 ldb);
-        if (e != hipSuccess) return hip_status(e, "gemm");
-      }
-    }
-    const int last = ((n - 1) / bs) * bs;
-    for (int k = last; k >= 0; k -= bs) {
-      const int kb = std::min(bs, n - k);
-      e = launch_trsm<T>(ctx, false, false, false, kb, nrhs, lu + (int64_t)k * lda + k, lda,
-                         b + (int64_t)k * ldb, ldb);
-      if (e != hipSuccess) return hip_status(e, "trsm");
-      if (k > 0) {
-        e = launch_gemm_minus<T>(ctx, mode, k, nrhs, kb, lu + k, lda, false, b + (int64_t)k * ldb,
-                                 ldb, b, ldb);
-        if (e != hipSuccess) return hip_status(e, "gemm");
-      }
-    }
-  } else {
-    for (int k = 0; k < n; k += bs) {
-      const int kb = std::min(bs, n - k);
-      e = launch_trsm<T>(ctx, false, true, false, kb, nrhs, lu + (int64_t)k * lda + k, lda,
-                         b + (int64_t)k * ldb, ldb);
-      if (e != hipSuccess) return hip_status(e, "trsm");
-      if (k + kb < n) {
-        // rows k+kb.. of U^T = (U[k..k+kb, k+kb..n])^T, stored K x M at lu[k][k+kb]
-        e = launch_gemm_minus<T>(ctx, mode, n - k - kb, nrhs, kb, lu + (int64_t)k * lda + k + kb,
-                                 lda, true, b + (int64_t)k * ldb, ldb, b + (int64_t)(k + kb) * ldb,
-                                 ldb);
-        if (e != hipSuccess) return hip_status(e, "gemm");
-      }
-    }
-    const int last = ((n - 1) / bs) * bs;
-    for (int k = last; k >= 0; k -= bs) {
-      const int kb = std::min(bs, n - k);
-      e = launch_trsm<T>(ctx, true, true, true, kb, nrhs, lu + (int64_t)k * lda + k, lda,
-                         b + (int64_t)k * ldb, ldb);
-      if (e != hipSuccess) return hip_status(e, "trsm");
-      if (k > 0) {
-        // rows 0..k of L^T = (L[k..k+kb, 0..k])^T, stored K x M at lu[k][0]
-        e = launch_gemm_minus<T>(ctx, mode, k, nrhs, kb, lu + (int64_t)k * lda, lda, true,
-                                 b + (int64_t)k * ldb, ldb, b, ldb);
-        if (e != hipSuccess) return hip_status(e, "gemm");
-      }
-    }
-    e = launch_laswp<T>(ctx, b, ldb, 0, nrhs, 0, 0, ipiv, 0, n, true);
-    if (e != hipSuccess) return hip_status(e, "laswp");
-  }
-  return LAIR_OK;
+  SolveAux<T> aux;
+  lair_status st = make_aux<T>(ctx, n, lu, lda, ipiv, nullptr, &aux);
+  if (st != LAIR_OK) return st;
+  return getrs_aux<T>(ctx, n, nrhs, lu, lda, aux, b, ldb, trans);
 }
 
 static double bits_to_double(unsigned long long b) {
@@ -410,15 +433,17 @@ lair_status lair_context_create(lair_context **out) {
   std::unique_ptr<lair_context> ctx(new lair_context());
   LAIR_HIP(hipGetDevice(&ctx->device));
   LAIR_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-  LAIR_HIP(hipMalloc(&ctx->flags, sizeof(uint32_t) * kMaxPanelWG));
-  LAIR_HIP(hipMalloc(&ctx->cand_val, sizeof(double) * 2 * kMaxPanelWG));
-  LAIR_HIP(hipMalloc(&ctx->cand_idx, sizeof(int32_t) * 2 * kMaxPanelWG));
-  LAIR_HIP(hipMalloc(&ctx->cand_row, sizeof(double) * 2 * kMaxPanelWG * kLeafW));
-  LAIR_HIP(hipMalloc(&ctx->diag_row, sizeof(double) * 2 * kLeafW));
+  LAIR_HIP(hipMalloc(&ctx->rec, sizeof(unsigned long long) * 2 * kMaxPanelWG * 64));
+  LAIR_HIP(hipMalloc(&ctx->drec, sizeof(unsigned long long) * 2 * 64));
   LAIR_HIP(hipMalloc(&ctx->err, sizeof(int32_t)));
   LAIR_HIP(hipMalloc(&ctx->sing, sizeof(int64_t)));
+  LAIR_HIP(hipMalloc(&ctx->swap_rows, sizeof(int) * kMaxSwapRows));
+  LAIR_HIP(hipMalloc(&ctx->swap_src, sizeof(int) * kMaxSwapRows));
+  LAIR_HIP(hipMalloc(&ctx->swap_count, sizeof(int)));
   LAIR_HIP(hipMalloc(&ctx->norms, sizeof(double) * 4096));
-  LAIR_HIP(launch_fill_u32(ctx.get(), ctx->flags, kMaxPanelWG, 0u));
+  LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->rec), 4 * kMaxPanelWG * 64, 0u));
+  LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->drec), 4 * 64, 0u));
+  ctx->seq = 1;
   LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->err), 1, 0u));
   LAIR_HIP(hipStreamSynchronize(ctx->stream));
   *out = ctx.release();
@@ -429,16 +454,17 @@ lair_status lair_context_destroy(lair_context *ctx) {
   if (!ctx) return -1;
   hipStreamSynchronize(ctx->stream);
   for (auto e : ctx->event_pool) hipEventDestroy(e);
-  hipFree(ctx->flags);
-  hipFree(ctx->cand_val);
-  hipFree(ctx->cand_idx);
-  hipFree(ctx->cand_row);
-  hipFree(ctx->diag_row);
+  hipFree(ctx->rec);
+  hipFree(ctx->drec);
   hipFree(ctx->err);
   hipFree(ctx->sing);
   hipFree(ctx->norms);
+  hipFree(ctx->swap_rows);
+  hipFree(ctx->swap_src);
+  hipFree(ctx->swap_count);
   for (Buffer *b : {&ctx->ws_a32, &ctx->ws_lu64, &ctx->ws_ipiv, &ctx->ws_x32, &ctx->ws_r,
-                    &ctx->ws_tmp, &ctx->ws_rhs})
+                    &ctx->ws_tmp, &ctx->ws_rhs, &ctx->ws_perm, &ctx->ws_linv, &ctx->ws_uinv,
+                    &ctx->ws_flags, &ctx->ws_tmp2})
     if (b->ptr) hipFree(b->ptr);
   hipStreamDestroy(ctx->stream);
   if (ctx == g_default) g_default = nullptr;
@@ -597,12 +623,14 @@ static lair_status dgesv_device_locked(lair_context *ctx, int64_t n, int64_t nrh
   const int64_t ld = pad_ld<double>(n);
   LAIR_HIP(ensure(ctx->ws_lu64, sizeof(double) * n * ld));
   LAIR_HIP(ensure(ctx->ws_ipiv, sizeof(int64_t) * n));
+  LAIR_HIP(ensure(ctx->ws_perm, sizeof(int) * n));
   double *lu = static_cast<double *>(ctx->ws_lu64.ptr);
   int64_t *piv = static_cast<int64_t *>(ctx->ws_ipiv.ptr);
+  int *perm = static_cast<int *>(ctx->ws_perm.ptr);
   hipError_t e = launch_copy_d2d(ctx, (int)n, (int)n, a, lda, lu, ld);
   if (e != hipSuccess) return hip_status(e, "copy");
   int64_t sing = -1;
-  lair_status st = getrf_impl<double>(ctx, n, n, lu, ld, piv, 0, kGemmF64, &sing);
+  lair_status st = getrf_impl<double>(ctx, n, n, lu, ld, piv, 0, kGemmF64, &sing, perm);
   if (st != LAIR_OK) return st;
   if (sing >= 0) {
     set_error("`a` is a singular matrix");
@@ -610,7 +638,11 @@ static lair_status dgesv_device_locked(lair_context *ctx, int64_t n, int64_t nrh
   }
   e = launch_copy_d2d(ctx, (int)n, (int)nrhs, b, ldb, x, ldx);
   if (e != hipSuccess) return hip_status(e, "copy");
-  return getrs_impl<double>(ctx, n, nrhs, lu, ld, piv, x, ldx, 0);
+  if (n <= kExactOrderMax) return getrs_impl<double>(ctx, n, nrhs, lu, ld, piv, x, ldx, 0);
+  SolveAux<double> aux;
+  st = make_aux<double>(ctx, (int)n, lu, ld, piv, perm, &aux);
+  if (st != LAIR_OK) return st;
+  return getrs_aux<double>(ctx, (int)n, (int)nrhs, lu, ld, aux, x, ldx, 0);
 }
 
 lair_status lair_dgesv_device(lair_context *ctx, int64_t n, int64_t nrhs, const double *a,
@@ -695,22 +727,44 @@ lair_status lair_dgesv_mixed_device(lair_context *ctx, int64_t n64, int64_t nrhs
   const int64_t ld32 = pad_ld<float>(n);
   LAIR_HIP(ensure(ctx->ws_a32, sizeof(float) * n * ld32));
   LAIR_HIP(ensure(ctx->ws_ipiv, sizeof(int64_t) * n));
+  LAIR_HIP(ensure(ctx->ws_perm, sizeof(int) * n));
   LAIR_HIP(ensure(ctx->ws_x32, sizeof(float) * n * nrhs));
   LAIR_HIP(ensure(ctx->ws_r, sizeof(double) * n * nrhs));
   float *a32 = static_cast<float *>(ctx->ws_a32.ptr);
   int64_t *piv = static_cast<int64_t *>(ctx->ws_ipiv.ptr);
+  int *perm = static_cast<int *>(ctx->ws_perm.ptr);
   float *d32 = static_cast<float *>(ctx->ws_x32.ptr);
   double *r = static_cast<double *>(ctx->ws_r.ptr);
   e = launch_fill_f64(ctx, ctx->norms, 1, 0.0);
   if (e != hipSuccess) return hip_status(e, "fill");
   e = launch_convert_d2s_norm(ctx, n, n, a, lda, a32, ld32, norms);
   if (e != hipSuccess) return hip_status(e, "convert");
-  lair_status st = getrf_impl<float>(ctx, n, n, a32, ld32, piv, 0, mode, nullptr);
+  lair_status st = getrf_impl<float>(ctx, n, n, a32, ld32, piv, 0, mode, nullptr, perm);
   if (st != LAIR_OK) return st;
-  // Initial solve in the factorization precision.
-  e = launch_copy_d2s(ctx, n, nrhs, b, ldb, d32, nrhs);
-  if (e != hipSuccess) return hip_status(e, "copy");
-  st = getrs_impl<float>(ctx, n, nrhs, a32, ld32, piv, d32, nrhs, 0);
+  SolveAux<float> aux;
+  if (n > kExactOrderMax) {
+    st = make_aux<float>(ctx, n, a32, ld32, piv, perm, &aux);
+    if (st != LAIR_OK) return st;
+  }
+  // d = (LU)^-1 P rhs in FP32: the row gather and the FP64->FP32 conversion
+  // are one pass.
+  auto lowprec_solve = [&](const double *rhs, int64_t ldr) -> lair_status {
+    if (n <= kExactOrderMax) {
+      hipError_t e2 = launch_copy_d2s(ctx, n, nrhs, rhs, ldr, d32, nrhs);
+      if (e2 != hipSuccess) return hip_status(e2, "copy");
+      return getrs_impl<float>(ctx, n, nrhs, a32, ld32, piv, d32, nrhs, 0);
+    }
+    LAIR_HIP(ensure(ctx->ws_tmp2, sizeof(float) * (size_t)n * nrhs));
+    float *t = static_cast<float *>(ctx->ws_tmp2.ptr);
+    hipError_t e2 = launch_permute_rows<double, float>(ctx, n, nrhs, aux.perm, rhs, ldr, t, nrhs, false);
+    if (e2 != hipSuccess) return hip_status(e2, "permute");
+    e2 = launch_trsv_dataflow<float>(ctx, a32, ld32, aux.linv, false, t, t, nrhs, n, nrhs, true, false);
+    if (e2 != hipSuccess) return hip_status(e2, "trsv");
+    e2 = launch_trsv_dataflow<float>(ctx, a32, ld32, aux.uinv, false, t, d32, nrhs, n, nrhs, false,
+                                     false);
+    return hip_status(e2, "trsv");
+  };
+  st = lowprec_solve(b, ldb);
   if (st != LAIR_OK) return st;
   e = launch_copy_s2d(ctx, n, nrhs, d32, nrhs, x, ldx);
   if (e != hipSuccess) return hip_status(e, "copy");
@@ -729,9 +783,7 @@ lair_status lair_dgesv_mixed_device(lair_context *ctx, int64_t n64, int64_t nrhs
     }
     if (it >= max_iter || !std::isfinite(be)) break;
     // Correction d = LU^-1 r in the factorization precision, x += d in FP64.
-    e = launch_copy_d2s(ctx, n, nrhs, r, nrhs, d32, nrhs);
-    if (e != hipSuccess) return hip_status(e, "copy");
-    st = getrs_impl<float>(ctx, n, nrhs, a32, ld32, piv, d32, nrhs, 0);
+    st = lowprec_solve(r, nrhs);
     if (st != LAIR_OK) return st;
     e = launch_axpy_s2d(ctx, n, nrhs, d32, nrhs, x, ldx);
     if (e != hipSuccess) return hip_status(e, "axpy");
@@ -754,15 +806,15 @@ lair_status lair_dgesv_mixed_device(lair_context *ctx, int64_t n64, int64_t nrhs
 
 // ---- instrumentation -----------------------------------------------------
 
-lair_status lair_profile_enable(lair_context *ctx, int32_t on) {
+lair_status lair_profile_enable(lair_context *ctx, int32_t class_mask) {
   if (!ctx) return -1;
   std::lock_guard<std::mutex> lk(ctx->mu);
-  ctx->profiling = on != 0;
+  ctx->prof_mask = class_mask;
   return LAIR_OK;
 }
 
 lair_status lair_profile_read(lair_context *ctx, int32_t kclass, double *ms, int64_t *launches,
-                              double *flops) {
+                              double *flops, double *bytes) {
   if (!ctx) return -1;
   if (kclass < 0 || kclass >= LAIR_KCLASS_COUNT) return -2;
   std::lock_guard<std::mutex> lk(ctx->mu);
@@ -770,6 +822,7 @@ lair_status lair_profile_read(lair_context *ctx, int32_t kclass, double *ms, int
   if (ms) *ms = ctx->prof_ms[kclass];
   if (launches) *launches = ctx->prof_n[kclass];
   if (flops) *flops = ctx->prof_flops[kclass];
+  if (bytes) *bytes = ctx->prof_bytes[kclass];
   return LAIR_OK;
 }
 
@@ -781,6 +834,7 @@ lair_status lair_profile_reset(lair_context *ctx) {
     ctx->prof_ms[i] = 0;
     ctx->prof_n[i] = 0;
     ctx->prof_flops[i] = 0;
+    ctx->prof_bytes[i] = 0;
   }
   return LAIR_OK;
 }

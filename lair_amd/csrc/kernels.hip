@@ -4,13 +4,19 @@
 //    few cooperating workgroups that synchronize once per column through
 //    per-workgroup sequence flags (src/lapack/getrf.rs:55-117, src/blas/iamax.rs:6-21)
 //  * laswp: row interchanges over column ranges (src/lapack/laswp.rs:11-40)
-//  * trsm: LDS-resident triangular solve of a block of right-hand sides
-//    (src/blas/trsm.rs:6-22, and the substitutions of src/lapack/getrs.rs:24-36)
+//  * getrs_exact: the reference's substitution order for small systems
+//    (src/lapack/getrs.rs:23-36)
 //  * gemm_minus: C -= A*B on MFMA (src/blas/gemm.rs:6-32 with alpha = -1)
 //  * FP64 residual, norms and conversions for iterative refinement.
 #include <climits>
 
 #include "lair_internal.h"
+
+// Phase marker inside the panel kernel: expands to nothing in the library;
+// a profiling build may define it to record a timestamp.
+#ifndef LAIR_PANEL_MARK
+#define LAIR_PANEL_MARK(col, phase)
+#endif
 
 namespace lair {
 
@@ -43,6 +49,9 @@ __device__ __forceinline__ void better(double &bv, int &br, double ov, int orr) 
 
 // ---------------------------------------------------------------- panel ---
 
+// Words per exchange record: value (2), row (1), row data (W * words per element).
+constexpr int kRecWords = 64;
+
 template <typename T>
 struct PanelArgs {
   T *a;
@@ -50,22 +59,88 @@ struct PanelArgs {
   int row0, m, col0, wcols, npiv, P;
   int64_t *ipiv;
   int64_t *sing;
-  uint32_t *flags;
+  unsigned long long *rec;   // [2][kMaxPanelWG][kRecWords] tagged words
+  unsigned long long *drec;  // [2][kRecWords] tagged words of the diagonal row
   uint32_t seq;
-  double *cand_val;
-  int32_t *cand_idx;
-  T *cand_row;
-  T *diag_row;
   int32_t *err;
 };
+
+__device__ __forceinline__ unsigned long long tagged(uint32_t tag, uint32_t data) {
+  return ((unsigned long long)tag << 32) | data;
+}
+
+template <typename T>
+__device__ __forceinline__ uint32_t word_of(T x, int part) {
+  if constexpr (sizeof(T) == 8) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    return part == 0 ? (uint32_t)(b >> 32) : (uint32_t)b;
+  } else {
+    return (uint32_t)__float_as_uint(x);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T from_words(const uint32_t *w) {
+  if constexpr (sizeof(T) == 8)
+    return __longlong_as_double((long long)(((unsigned long long)w[0] << 32) | w[1]));
+  else
+    return __uint_as_float(w[0]);
+}
+
+// DPP move of a double (two 32-bit halves); lanes without a source read 0.
+template <int CTRL, int ROWM = 0xF>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const int lo = __double2loint(x), hi = __double2hiint(x);
+  const int l2 = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROWM, 0xF, false);
+  const int h2 = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROWM, 0xF, false);
+  return __hiloint2double(h2, l2);
+}
+
+__device__ __forceinline__ double readlane_f64(double x, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), lane);
+  return __hiloint2double(hi, lo);
+}
+
+// Maximum of non-negative doubles over the 64 lanes: quad swaps, half-row and
+// row mirrors, then row broadcasts 15 and 31 leave the maximum in lane 63.
+__device__ __forceinline__ double wave_max_nonneg(double v) {
+  v = fmax(v, dpp_f64<0xB1>(v));        // quad_perm [1,0,3,2]
+  v = fmax(v, dpp_f64<0x4E>(v));        // quad_perm [2,3,0,1]
+  v = fmax(v, dpp_f64<0x141>(v));       // row_half_mirror
+  v = fmax(v, dpp_f64<0x140>(v));       // row_mirror
+  v = fmax(v, dpp_f64<0x142, 0xA>(v));  // row_bcast:15 -> rows 1, 3
+  v = fmax(v, dpp_f64<0x143, 0xC>(v));  // row_bcast:31 -> rows 2, 3
+  return readlane_f64(v, 63);
+}
+
+// iamax over one wave: each lane holds (value, row) with value > 0 for a
+// candidate and (0, INT_MAX) otherwise.  Returns the largest value and, among
+// the lanes holding it, the lowest row (uniform across the wave).
+__device__ __forceinline__ void wave_argmax(double v, int row, double &bv, int &br) {
+  const double vmax = wave_max_nonneg(v);
+  const unsigned long long m = __ballot(row != INT_MAX && v == vmax);
+  int best = INT_MAX;
+  unsigned long long mm = m;
+  while (mm) {
+    const int l = __ffsll((long long)mm) - 1;
+    mm &= mm - 1;
+    const int rl = __builtin_amdgcn_readlane(row, l);
+    best = rl < best ? rl : best;
+  }
+  bv = m ? vmax : 0.0;
+  br = best;
+}
 
 // Factors columns [col0, col0+wcols) of rows [row0, m) in place.  Workgroup g
 // owns rows [row0 + g*R, row0 + (g+1)*R) with R = NT*RPT; thread t keeps rows
 // base + t + s*NT (s < RPT) of the leaf in registers.  Per column j:
-//  1. local argmax over the rows at or below the diagonal d = row0 + j;
-//  2. with P > 1 workgroups: publish (value, row, row data), raise this
-//     workgroup's sequence flag, wait for every peer's flag, and reduce the P
-//     candidates in the same order everywhere;
+//  1. argmax over the rows at or below the diagonal d = row0 + j: per thread,
+//     per wave (DPP) and across the waves of the workgroup (LDS);
+//  2. with P > 1 workgroups: every workgroup publishes one record (value,
+//     row, row data) whose 64-bit words each carry the column's tag, so a
+//     reader that sees every tag also has every datum -- one round trip;
+//     all records are reduced in the same order everywhere;
 //  3. swap the diagonal row and the pivot row, record ipiv[d] (and the
 //     singular column when no candidate exists), scale by the reciprocal of
 //     the pivot and apply the rank-1 update with separate multiply and
@@ -77,15 +152,87 @@ template <typename T, int W, int RPT, int NT>
 struct PanelLeaf {
   static constexpr int NWAVE = NT / 64;
   static constexpr int R = NT * RPT;
+  static constexpr int WPE = sizeof(T) / 4;  // 32-bit words per element
+  static constexpr int L = 3 + W * WPE;      // record length in words
+  static_assert(L <= kRecWords, "record too long");
   const PanelArgs<T> &pa;
   T (&v)[RPT][W];
-  double *s_wv;
-  int *s_wr;
-  T *s_prow;
-  T *s_drow;
-  int *s_win;
+  double (*s_wv)[NWAVE];      // [2][NWAVE] wave maxima, by column parity
+  int (*s_wr)[NWAVE];         // [2][NWAVE] wave rows
+  T (*s_wrow)[NWAVE][W];      // [2][NWAVE][W] wave winners' rows
+  T (*s_prow)[W];             // [2][W] pivot row
+  T (*s_drow)[W];             // [2][W] diagonal row before the interchange
+  int *s_win;                 // [2]
+  uint32_t *s_rec;            // [P][L] record words, then [W*WPE] diagonal words
   int *s_bail;
   int g, t, lane, wave, base;
+
+  template <int J>
+  __device__ __forceinline__ bool exchange(double bv, int br, int qw) {
+    constexpr int par = J & 1;
+    const uint32_t tg = pa.seq + (uint32_t)J + 1u;
+    if (wave == 0 && lane < L) {
+      uint32_t data;
+      if (lane == 0)
+        data = word_of<double>(bv, 0);
+      else if (lane == 1)
+        data = word_of<double>(bv, 1);
+      else if (lane == 2)
+        data = (uint32_t)br;
+      else
+        data = br == INT_MAX ? 0u : word_of<T>(s_wrow[par][qw][(lane - 3) / WPE], (lane - 3) % WPE);
+      st_agent(pa.rec + ((size_t)par * pa.P + g) * kRecWords + lane, tagged(tg, data));
+    }
+    if (g == 0 && wave == NWAVE - 1 && lane < W * WPE)
+      st_agent(pa.drec + (size_t)par * kRecWords + lane,
+               tagged(tg, word_of<T>(s_drow[par][lane / WPE], lane % WPE)));
+    if (wave == 0) {
+      const int total = pa.P * L;
+      int spins = 0;
+      bool bail = false;
+      for (;;) {
+        bool ok = true;
+        for (int idx = lane; idx < total; idx += 64) {
+          const int q = idx / L, w = idx % L;
+          const unsigned long long x = ld_agent(pa.rec + ((size_t)par * pa.P + q) * kRecWords + w);
+          ok = ok && (uint32_t)(x >> 32) == tg;
+          s_rec[idx] = (uint32_t)x;
+        }
+        if (lane < W * WPE) {
+          const unsigned long long x = ld_agent(pa.drec + (size_t)par * kRecWords + lane);
+          ok = ok && (uint32_t)(x >> 32) == tg;
+          s_rec[total + lane] = (uint32_t)x;
+        }
+        if (__all(ok)) break;
+        if (++spins > kSpinLimit || ld_agent(pa.err) != 0) {
+          bail = true;
+          break;
+        }
+      }
+      double cv = 0.0;
+      int cr = INT_MAX;
+      if (!bail && lane < pa.P) {
+        const uint32_t *rw = s_rec + lane * L;
+        cv = from_words<double>(rw);
+        cr = (int)rw[2];
+      }
+      double wv;
+      int wr;
+      wave_argmax(cv, cr, wv, wr);
+      if (wr != INT_MAX && lane < W) {
+        const int q = (wr - pa.row0) / R;
+        s_prow[par][lane] = from_words<T>(s_rec + q * L + 3 + lane * WPE);
+      }
+      if (lane < W) s_drow[par][lane] = from_words<T>(s_rec + total + lane * WPE);
+      if (lane == 0) {
+        s_win[par] = wr;
+        *s_bail = bail ? 1 : 0;
+        if (bail) st_agent(pa.err, 1);
+      }
+    }
+    __syncthreads();
+    return *s_bail == 0;
+  }
 
   template <int J>
   __device__ __forceinline__ bool step() {
@@ -106,113 +253,52 @@ struct PanelLeaf {
         }
       }
     }
+    double wv;
+    int wr;
+    wave_argmax(bv, br, wv, wr);
+    LAIR_PANEL_MARK(J, 1);
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      const double ov = __shfl_xor(bv, off);
-      const int orr = __shfl_xor(br, off);
-      better(bv, br, ov, orr);
+    for (int s = 0; s < RPT; ++s) {
+      const int r = base + t + s * NT;
+      if (r == wr) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) s_wrow[par][wave][w] = v[s][w];
+      }
+      if (r == d) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) s_drow[par][w] = v[s][w];
+      }
     }
     if (lane == 0) {
-      s_wv[wave] = bv;
-      s_wr[wave] = br;
+      s_wv[par][wave] = wv;
+      s_wr[par][wave] = wr;
     }
-    if (t == 0) *s_bail = 0;
     __syncthreads();
-    bv = s_wv[0];
-    br = s_wr[0];
+    LAIR_PANEL_MARK(J, 2);
+    bv = s_wv[par][0];
+    br = s_wr[par][0];
+    int qw = 0;
 #pragma unroll
-    for (int q = 1; q < NWAVE; ++q) better(bv, br, s_wv[q], s_wr[q]);
-
-    if (pa.P == 1) {
-#pragma unroll
-      for (int s = 0; s < RPT; ++s) {
-        const int r = base + t + s * NT;
-        if (r == br) {
-#pragma unroll
-          for (int w = 0; w < W; ++w) s_prow[w] = v[s][w];
-        }
-        if (r == d) {
-#pragma unroll
-          for (int w = 0; w < W; ++w) s_drow[w] = v[s][w];
-        }
+    for (int q = 1; q < NWAVE; ++q) {
+      const double ov = s_wv[par][q];
+      const int orr = s_wr[par][q];
+      if (ov > bv || (ov == bv && orr < br)) {
+        bv = ov;
+        br = orr;
+        qw = q;
       }
-      if (t == 0) *s_win = br;
-      __syncthreads();
-    } else {
-      T *crow = pa.cand_row + ((size_t)par * pa.P + g) * W;
-      bool wrote = false;
-#pragma unroll
-      for (int s = 0; s < RPT; ++s) {
-        const int r = base + t + s * NT;
-        if (r == br) {
-#pragma unroll
-          for (int w = 0; w < W; ++w) st_agent(crow + w, v[s][w]);
-          wrote = true;
-        }
-        if (r == d) {
-#pragma unroll
-          for (int w = 0; w < W; ++w) st_agent(pa.diag_row + par * W + w, v[s][w]);
-          wrote = true;
-        }
-      }
-      if (t == 0) {
-        st_agent(pa.cand_val + par * pa.P + g, bv);
-        st_agent(pa.cand_idx + par * pa.P + g, br);
-        wrote = true;
-      }
-      if (wrote) __threadfence();
-      __syncthreads();
-      const uint32_t target = pa.seq + (uint32_t)J + 1u;
-      if (t == 0)
-        __hip_atomic_store(pa.flags + g, target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      if (wave == 0) {
-        int spins = 0;
-        bool bail = false;
-        for (;;) {
-          uint32_t f = target;
-          if (lane < pa.P)
-            f = __hip_atomic_load(pa.flags + lane, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-          const bool ok = (int32_t)(f - target) >= 0;
-          if (__all(ok)) break;
-          if (++spins > kSpinLimit || ld_agent(pa.err) != 0) {
-            bail = true;
-            break;
-          }
-        }
-        double cv = 0.0;
-        int cr = INT_MAX;
-        if (!bail && lane < pa.P) {
-          cv = ld_agent(pa.cand_val + par * pa.P + lane);
-          cr = ld_agent(pa.cand_idx + par * pa.P + lane);
-        }
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-          const double ov = __shfl_xor(cv, off);
-          const int orr = __shfl_xor(cr, off);
-          better(cv, cr, ov, orr);
-        }
-        if (lane == 0) {
-          *s_win = cr;
-          if (bail) {
-            *s_bail = 1;
-            st_agent(pa.err, 1);
-          }
-        }
-      }
-      __syncthreads();
-      if (*s_bail) return false;
-      const int win = *s_win;
-      if (t < W) {
-        if (win != INT_MAX) {
-          const int gw = (win - pa.row0) / R;
-          s_prow[t] = ld_agent(pa.cand_row + ((size_t)par * pa.P + gw) * W + t);
-        }
-        s_drow[t] = ld_agent(pa.diag_row + par * W + t);
-      }
-      __syncthreads();
     }
-
-    const int win = *s_win;
+    const T *prow;
+    int win;
+    if (pa.P == 1) {
+      win = br;
+      prow = s_wrow[par][qw];
+    } else {
+      if (!exchange<J>(bv, br, qw)) return false;
+      win = s_win[par];
+      prow = s_prow[par];
+    }
+    LAIR_PANEL_MARK(J, 3);
     const bool singular = (win == INT_MAX);
     const int pr = singular ? d : win;
     if (pr != d) {
@@ -221,10 +307,10 @@ struct PanelLeaf {
         const int r = base + t + s * NT;
         if (r == d) {
 #pragma unroll
-          for (int w = 0; w < W; ++w) v[s][w] = s_prow[w];
+          for (int w = 0; w < W; ++w) v[s][w] = prow[w];
         } else if (r == pr) {
 #pragma unroll
-          for (int w = 0; w < W; ++w) v[s][w] = s_drow[w];
+          for (int w = 0; w < W; ++w) v[s][w] = s_drow[par][w];
         }
       }
     }
@@ -233,7 +319,7 @@ struct PanelLeaf {
       if (singular) *pa.sing = d;
     }
     if (!singular) {
-      const T recip = T(1) / s_prow[J];
+      const T recip = T(1) / prow[J];
 #pragma unroll
       for (int s = 0; s < RPT; ++s) {
         const int r = base + t + s * NT;
@@ -242,13 +328,16 @@ struct PanelLeaf {
           v[s][J] = l;
 #pragma unroll
           for (int k = J + 1; k < W; ++k) {
-            const T e = l * s_prow[k];
+            const T e = l * prow[k];
             v[s][k] = v[s][k] - e;
           }
         }
       }
     }
-    __syncthreads();
+    LAIR_PANEL_MARK(J, 4);
+    // No trailing barrier: the next column uses the other parity's buffers,
+    // and the one after that is written only after the next column's first
+    // barrier, which every thread reaches after finishing this update.
     return true;
   }
 };
@@ -273,6 +362,7 @@ template <typename T, int W, int RPT, int NT>
 __global__ void __launch_bounds__(NT) panel_coop_kernel(PanelArgs<T> pa) {
   constexpr int NWAVE = NT / 64;
   constexpr int R = NT * RPT;
+  constexpr int L = PanelLeaf<T, W, RPT, NT>::L;
   const int g = blockIdx.x, t = threadIdx.x;
   const int base = pa.row0 + g * R;
 
@@ -285,16 +375,20 @@ __global__ void __launch_bounds__(NT) panel_coop_kernel(PanelArgs<T> pa) {
     for (int w = 0; w < W; ++w) v[s][w] = (r < pa.m && w < pa.wcols) ? src[w] : T(0);
   }
 
-  __shared__ double s_wv[NWAVE];
-  __shared__ int s_wr[NWAVE];
-  __shared__ T s_prow[W];
-  __shared__ T s_drow[W];
-  __shared__ int s_win;
+  __shared__ double s_wv[2][NWAVE];
+  __shared__ int s_wr[2][NWAVE];
+  __shared__ T s_wrow[2][NWAVE][W];
+  __shared__ T s_prow[2][W];
+  __shared__ T s_drow[2][W];
+  __shared__ int s_win[2];
+  __shared__ uint32_t s_rec[kMaxPanelWG * L + W * (sizeof(T) / 4)];
   __shared__ int s_bail;
 
-  PanelLeaf<T, W, RPT, NT> leaf{pa, v, s_wv, s_wr, s_prow, s_drow, &s_win, &s_bail,
+  PanelLeaf<T, W, RPT, NT> leaf{pa, v, s_wv, s_wr, s_wrow, s_prow, s_drow, s_win, s_rec, &s_bail,
                                 g, t, t & 63, t >> 6, base};
+  LAIR_PANEL_MARK(0, 0);
   if (!ColumnLoop<0, W>::run(leaf)) return;
+  LAIR_PANEL_MARK(W, 0);
 
 #pragma unroll
   for (int s = 0; s < RPT; ++s) {
@@ -323,12 +417,9 @@ hipError_t launch_panel(lair_context *ctx, T *a, int64_t lda, int row0, int m, i
   pa.npiv = wcols < rows ? wcols : rows;
   pa.ipiv = ipiv;
   pa.sing = ctx->sing;
-  pa.flags = ctx->flags;
+  pa.rec = ctx->rec;
+  pa.drec = ctx->drec;
   pa.seq = ctx->seq;
-  pa.cand_val = ctx->cand_val;
-  pa.cand_idx = ctx->cand_idx;
-  pa.cand_row = reinterpret_cast<T *>(ctx->cand_row);
-  pa.diag_row = reinterpret_cast<T *>(ctx->diag_row);
   pa.err = ctx->err;
   constexpr int NT = 256;
   int rpt;
@@ -341,7 +432,7 @@ hipError_t launch_panel(lair_context *ctx, T *a, int64_t lda, int row0, int m, i
   const int R = NT * rpt;
   pa.P = (rows + R - 1) / R;
   if (pa.P > kMaxPanelWG) return hipErrorInvalidValue;
-  const int slot = prof_begin(ctx);
+  const int slot = prof_begin(ctx, LAIR_KCLASS_PANEL);
   if (rpt == 1)
     panel_coop_kernel<T, kLeafW, 1, NT><<<pa.P, NT, 0, ctx->stream>>>(pa);
   else if (rpt == 2)
@@ -349,7 +440,7 @@ hipError_t launch_panel(lair_context *ctx, T *a, int64_t lda, int row0, int m, i
   else
     panel_coop_kernel<T, kLeafW, 4, NT><<<pa.P, NT, 0, ctx->stream>>>(pa);
   const double fl = (double)pa.npiv * (double)rows * (double)wcols;
-  prof_end(ctx, slot, LAIR_KCLASS_PANEL, fl);
+  prof_end(ctx, slot, LAIR_KCLASS_PANEL, fl, 2.0 * sizeof(T) * rows * wcols);
   ctx->seq += (uint32_t)kLeafW;
   return hipGetLastError();
 }
@@ -401,10 +492,10 @@ hipError_t launch_laswp(lair_context *ctx, T *a, int64_t lda, int c0, int c1, in
   const int n1 = s0 - c0;
   const int total = n1 + (c1 - s1);
   if (total <= 0 || k1 <= k0) return hipSuccess;
-  const int slot = prof_begin(ctx);
+  const int slot = prof_begin(ctx, LAIR_KCLASS_LASWP);
   laswp_kernel<T><<<(total + 255) / 256, 256, 0, ctx->stream>>>(a, lda, c0, n1, s1, total, ipiv,
                                                                  k0, k1, rev ? 1 : 0);
-  prof_end(ctx, slot, LAIR_KCLASS_LASWP, 0.0);
+  prof_end(ctx, slot, LAIR_KCLASS_LASWP, 0.0, 4.0 * sizeof(T) * (double)(k1 - k0) * total);
   return hipGetLastError();
 }
 
@@ -412,74 +503,6 @@ template hipError_t launch_laswp<double>(lair_context *, double *, int64_t, int,
                                          const int64_t *, int, int, bool);
 template hipError_t launch_laswp<float>(lair_context *, float *, int64_t, int, int, int, int,
                                         const int64_t *, int, int, bool);
-
-// ----------------------------------------------------------------- trsm ---
-
-// One workgroup solves a kb x 64 block of right-hand sides held in LDS.
-// Forward substitution (effective lower triangle) or backward (effective
-// upper).  eff(i,p) = trans ? tri[p][i] : tri[i][p].  After x_i is final its
-// contribution is subtracted from every later row (right-looking order);
-// a non-unit diagonal uses true division like getrs.rs:35.
-template <typename T>
-__global__ void __launch_bounds__(256)
-    trsm_kernel(int kb, int ncols, const T *tri, int64_t ldt, T *b, int64_t ldb, int forward,
-                int trans, int unit) {
-#pragma clang fp contract(off)
-  extern __shared__ unsigned char smem_raw[];
-  T *xs = reinterpret_cast<T *>(smem_raw);  // [kb][64]
-  const int t = threadIdx.x;
-  const int col = t & 63, rg = t >> 6;
-  const int c = blockIdx.x * 64 + col;
-  const bool active = c < ncols;
-  for (int i = rg; i < kb; i += 4) xs[i * 64 + col] = active ? b[(int64_t)i * ldb + c] : T(0);
-  __syncthreads();
-  if (forward) {
-    for (int i = 0; i < kb; ++i) {
-      if (!unit) {
-        if (rg == 0) xs[i * 64 + col] = xs[i * 64 + col] / tri[(int64_t)i * ldt + i];
-        __syncthreads();
-      }
-      const T xi = xs[i * 64 + col];
-      for (int r = i + 1 + rg; r < kb; r += 4) {
-        const T l = trans ? tri[(int64_t)i * ldt + r] : tri[(int64_t)r * ldt + i];
-        const T e = l * xi;
-        xs[r * 64 + col] = xs[r * 64 + col] - e;
-      }
-      __syncthreads();
-    }
-  } else {
-    for (int i = kb - 1; i >= 0; --i) {
-      if (!unit) {
-        if (rg == 0) xs[i * 64 + col] = xs[i * 64 + col] / tri[(int64_t)i * ldt + i];
-        __syncthreads();
-      }
-      const T xi = xs[i * 64 + col];
-      for (int r = rg; r < i; r += 4) {
-        const T u = trans ? tri[(int64_t)i * ldt + r] : tri[(int64_t)r * ldt + i];
-        const T e = u * xi;
-        xs[r * 64 + col] = xs[r * 64 + col] - e;
-      }
-      __syncthreads();
-    }
-  }
-  if (active)
-    for (int i = rg; i < kb; i += 4) b[(int64_t)i * ldb + c] = xs[i * 64 + col];
-}
-
-template <typename T>
-hipError_t launch_trsm(lair_context *ctx, bool lower, bool trans, bool unit, int kb, int ncols,
-                       const T *tri, int64_t ldt, T *b, int64_t ldb) {
-  if (kb <= 0 || ncols <= 0) return hipSuccess;
-  const size_t shm = (size_t)kb * 64 * sizeof(T);
-  if (shm > 160 * 1024) return hipErrorInvalidValue;
-  const bool forward = (lower && !trans) || (!lower && trans);
-  const int slot = prof_begin(ctx);
-  trsm_kernel<T><<<(ncols + 63) / 64, 256, shm, ctx->stream>>>(kb, ncols, tri, ldt, b, ldb,
-                                                               forward ? 1 : 0, trans ? 1 : 0,
-                                                               unit ? 1 : 0);
-  prof_end(ctx, slot, LAIR_KCLASS_TRSM, (double)kb * kb * ncols);
-  return hipGetLastError();
-}
 
 // Small systems (n <= kExactOrderMax): getrs in the reference's exact
 // operation order (getrs.rs:23-36): forward interchanges, x[i] -= a[i,k]*x[k]
@@ -529,7 +552,7 @@ hipError_t launch_getrs_exact(lair_context *ctx, int n, int nrhs, const T *lu, i
   if (n <= 0 || nrhs <= 0) return hipSuccess;
   if (n > kExactOrderMax) return hipErrorInvalidValue;
   const size_t shm = (size_t)n * 64 * sizeof(T);
-  const int slot = prof_begin(ctx);
+  const int slot = prof_begin(ctx, LAIR_KCLASS_GETRS);
   getrs_exact_kernel<T><<<(nrhs + 63) / 64, 64, shm, ctx->stream>>>(n, nrhs, lu, lda, ipiv, b, ldb);
   prof_end(ctx, slot, LAIR_KCLASS_GETRS, 2.0 * n * n * nrhs);
   return hipGetLastError();
@@ -539,11 +562,6 @@ template hipError_t launch_getrs_exact<double>(lair_context *, int, int, const d
                                                const int64_t *, double *, int64_t);
 template hipError_t launch_getrs_exact<float>(lair_context *, int, int, const float *, int64_t,
                                               const int64_t *, float *, int64_t);
-
-template hipError_t launch_trsm<double>(lair_context *, bool, bool, bool, int, int, const double *,
-                                        int64_t, double *, int64_t);
-template hipError_t launch_trsm<float>(lair_context *, bool, bool, bool, int, int, const float *,
-                                       int64_t, float *, int64_t);
 
 // ----------------------------------------------------------------- gemm ---
 
@@ -792,26 +810,34 @@ __global__ void __launch_bounds__(256)
 template <>
 hipError_t launch_gemm_minus<double>(lair_context *ctx, int mode, int M, int N, int K,
                                      const double *A, int64_t lda, bool transA, const double *B,
-                                     int64_t ldb, double *C, int64_t ldc) {
+                                     int64_t ldb, double *C, int64_t ldc, int kclass) {
   if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
   (void)mode;
+  if (M >= 256 && N >= 128 && K >= 32) {
+    const int slot = prof_begin(ctx, kclass);
+    hipError_t e = launch_dgemm_tile(ctx, M, N, K, A, lda, transA, B, ldb, C, ldc);
+    prof_end(ctx, slot, kclass, 2.0 * M * N * K,
+             (double)sizeof(double) * (2.0 * M * N + (double)M * K + (double)K * N));
+    return e;
+  }
   dim3 grid((N + 63) / 64, (M + 63) / 64);
-  const int slot = prof_begin(ctx);
+  const int slot = prof_begin(ctx, kclass);
   if (transA)
     gemm_f64_kernel<true><<<grid, 256, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C, ldc);
   else
     gemm_f64_kernel<false><<<grid, 256, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C, ldc);
-  prof_end(ctx, slot, LAIR_KCLASS_GEMM, 2.0 * M * N * K);
+  prof_end(ctx, slot, kclass, 2.0 * M * N * K,
+           (double)sizeof(double) * (2.0 * M * N + (double)M * K + (double)K * N));
   return hipGetLastError();
 }
 
 template <>
 hipError_t launch_gemm_minus<float>(lair_context *ctx, int mode, int M, int N, int K,
                                     const float *A, int64_t lda, bool transA, const float *B,
-                                    int64_t ldb, float *C, int64_t ldc) {
+                                    int64_t ldb, float *C, int64_t ldc, int kclass) {
   if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
   dim3 grid((N + 63) / 64, (M + 63) / 64);
-  const int slot = prof_begin(ctx);
+  const int slot = prof_begin(ctx, kclass);
   hipStream_t st = ctx->stream;
   if (mode == kGemmBF16x6) {
     if (transA)
@@ -829,7 +855,8 @@ hipError_t launch_gemm_minus<float>(lair_context *ctx, int mode, int M, int N, i
     else
       gemm_f32_kernel<false><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
   }
-  prof_end(ctx, slot, LAIR_KCLASS_GEMM, 2.0 * M * N * K);
+  prof_end(ctx, slot, kclass, 2.0 * M * N * K,
+           (double)sizeof(float) * (2.0 * M * N + (double)M * K + (double)K * N));
   return hipGetLastError();
 }
 
@@ -885,7 +912,7 @@ hipError_t launch_convert_d2s_norm(lair_context *ctx, int n, int ncols, const do
                                    int64_t lda, float *a32, int64_t lda32,
                                    unsigned long long *norm_bits) {
   if (n <= 0) return hipSuccess;
-  const int slot = prof_begin(ctx);
+  const int slot = prof_begin(ctx, LAIR_KCLASS_CONVERT);
   convert_d2s_norm_kernel<<<(n + 3) / 4, 256, 0, ctx->stream>>>(n, ncols, a, lda, a32, lda32,
                                                                 norm_bits);
   prof_end(ctx, slot, LAIR_KCLASS_CONVERT, 0.0);
@@ -982,7 +1009,7 @@ hipError_t launch_residual(lair_context *ctx, int n, int nrhs, const double *a, 
                            int64_t ldr) {
   if (n <= 0 || nrhs <= 0) return hipSuccess;
   if (nrhs <= 4) {
-    const int slot = prof_begin(ctx);
+    const int slot = prof_begin(ctx, LAIR_KCLASS_RESIDUAL);
     const unsigned grid = (n + 3) / 4;
     for (int q0 = 0; q0 < nrhs; q0 += 4) {
       const int nr = nrhs - q0 < 4 ? nrhs - q0 : 4;

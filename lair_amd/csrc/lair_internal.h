@@ -21,6 +21,22 @@ constexpr int kLeafW = 16;
 constexpr int kExactOrderMax = 128;
 // Maximum number of workgroups in a cooperative panel launch.
 constexpr int kMaxPanelWG = 64;
+// Row interchanges of one panel touch at most 2 * (panel width) rows.
+constexpr int kMaxSwapRows = 512;
+// Largest n whose row permutation is composed in LDS.
+constexpr int kPermLdsMax = 32768;
+
+// Diagonal block size of the dataflow triangular solve.
+template <typename T>
+struct TrsvBlock;
+template <>
+struct TrsvBlock<double> {
+  static constexpr int BS = 64;
+};
+template <>
+struct TrsvBlock<float> {
+  static constexpr int BS = 128;
+};
 
 // Trailing-update arithmetic.
 enum GemmMode : int {
@@ -40,6 +56,7 @@ struct Buffer {
 struct ProfRecord {
   int kclass;
   double flops;
+  double bytes;
   hipEvent_t start, stop;
 };
 
@@ -51,33 +68,38 @@ struct lair_context {
   std::mutex mu;  // serializes calls that share this context's workspace
 
   // Cooperative panel synchronization state (device memory).
-  uint32_t *flags = nullptr;   // one monotonic sequence word per workgroup
-  uint32_t seq = 0;            // next sequence base
-  double *cand_val = nullptr;  // [2][kMaxPanelWG]
-  int32_t *cand_idx = nullptr; // [2][kMaxPanelWG]
-  double *cand_row = nullptr;  // [2][kMaxPanelWG][kLeafW] (holds float rows too)
-  double *diag_row = nullptr;  // [2][kLeafW]
+  // Tagged exchange records: [2][kMaxPanelWG][64] and [2][64] 64-bit words.
+  unsigned long long *rec = nullptr;
+  unsigned long long *drec = nullptr;
+  uint32_t seq = 0;            // tag base of the next panel launch
   int32_t *err = nullptr;      // set when a bounded wait expires
   int64_t *sing = nullptr;     // last zero-pivot column (device)
   double *norms = nullptr;     // scratch for norm reductions (uint64-ordered doubles)
 
+  // Row-interchange map of the current panel (device).
+  int *swap_rows = nullptr, *swap_src = nullptr, *swap_count = nullptr;
+
   // Grow-only workspaces.
   lair::Buffer ws_a32, ws_lu64, ws_ipiv, ws_x32, ws_r, ws_tmp, ws_rhs;
+  lair::Buffer ws_perm, ws_linv, ws_uinv, ws_flags, ws_tmp2;
+  size_t ws_flags_zeroed = 0;
+  uint32_t tri_epoch = 0;
 
-  bool profiling = false;
+  int32_t prof_mask = 0;  // bit k: record kernel class k
   std::vector<lair::ProfRecord> prof;
   std::vector<hipEvent_t> event_pool;
   size_t event_next = 0;
   double prof_ms[LAIR_KCLASS_COUNT] = {};
   int64_t prof_n[LAIR_KCLASS_COUNT] = {};
   double prof_flops[LAIR_KCLASS_COUNT] = {};
+  double prof_bytes[LAIR_KCLASS_COUNT] = {};
 };
 
 namespace lair {
 
 // Records (when profiling) a start event for a kernel of class kclass.
-int prof_begin(lair_context *ctx);
-void prof_end(lair_context *ctx, int slot, int kclass, double flops);
+int prof_begin(lair_context *ctx, int kclass);
+void prof_end(lair_context *ctx, int slot, int kclass, double flops, double bytes = 0.0);
 
 hipError_t ensure(Buffer &b, size_t bytes);
 
@@ -93,12 +115,6 @@ template <typename T>
 hipError_t launch_laswp(lair_context *ctx, T *a, int64_t lda, int c0, int c1, int s0, int s1,
                         const int64_t *ipiv, int k0, int k1, bool rev);
 
-// B := op(Tri)^-1 B.  lower/upper refers to the stored triangle; trans uses
-// its transpose; unit skips the diagonal (true division otherwise).
-template <typename T>
-hipError_t launch_trsm(lair_context *ctx, bool lower, bool trans, bool unit, int kb, int ncols,
-                       const T *tri, int64_t ldt, T *b, int64_t ldb);
-
 template <typename T>
 hipError_t launch_getrs_exact(lair_context *ctx, int n, int nrhs, const T *lu, int64_t lda,
                               const int64_t *ipiv, T *b, int64_t ldb);
@@ -106,7 +122,34 @@ hipError_t launch_getrs_exact(lair_context *ctx, int n, int nrhs, const T *lu, i
 // C[M x N] -= op(A)[M x K] * B[K x N]; transA reads A stored as K x M.
 template <typename T>
 hipError_t launch_gemm_minus(lair_context *ctx, int mode, int M, int N, int K, const T *A,
-                             int64_t lda, bool transA, const T *B, int64_t ldb, T *C, int64_t ldc);
+                             int64_t lda, bool transA, const T *B, int64_t ldb, T *C, int64_t ldc,
+                             int kclass = LAIR_KCLASS_GEMM);
+
+// Interchanges ipiv[k0..k1) on columns [c0,c1) minus [s0,s1), as one gather;
+// perm (optional) is updated with the same row map.
+template <typename T>
+hipError_t launch_rowswap(lair_context *ctx, T *a, int64_t lda, int c0, int c1, int s0, int s1,
+                          const int64_t *ipiv, int k0, int k1, int *perm);
+hipError_t launch_perm_from_ipiv(lair_context *ctx, const int64_t *ipiv, int n, int *perm);
+hipError_t launch_iota(lair_context *ctx, int *p, int n);
+template <typename S, typename D>
+hipError_t launch_permute_rows(lair_context *ctx, int rows, int cols, const int *perm, const S *x,
+                               int64_t ldx, D *y, int64_t ldy, bool scatter);
+// B := L^-1 B (unit lower L, kb <= 256).
+template <typename T>
+hipError_t launch_trsm_llu(lair_context *ctx, int kb, int ncols, const T *L, int64_t ldl, T *B,
+                           int64_t ldb);
+// Inverses of the diagonal blocks (TrsvBlock<T>::BS) of L (unit) and U.
+template <typename T>
+hipError_t launch_trtri_diag(lair_context *ctx, const T *lu, int64_t lda, int n, T *linv, T *uinv);
+template <typename T>
+hipError_t launch_trsv_dataflow(lair_context *ctx, const T *lu, int64_t lda, const T *dinv,
+                                bool dinv_trans, const T *xin, T *x, int64_t ldx, int n, int nrhs,
+                                bool forward, bool trans);
+
+// 128x128-tile MFMA trailing updates (gemm_kernels.hip).
+hipError_t launch_dgemm_tile(lair_context *ctx, int M, int N, int K, const double *A, int64_t lda,
+                             bool transA, const double *B, int64_t ldb, double *C, int64_t ldc);
 
 hipError_t launch_fill_i64(lair_context *ctx, int64_t *p, int64_t n, int64_t v);
 hipError_t launch_fill_u32(lair_context *ctx, uint32_t *p, int64_t n, uint32_t v);

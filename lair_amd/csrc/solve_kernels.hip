@@ -1,0 +1,617 @@
+// Row interchanges as gathers, blocked triangular solves and the dataflow
+// triangular solve used by getrs.
+//
+//  * rowmap + rowgather: the interchanges ipiv[k0..k1) applied in forward
+//    order (src/lapack/laswp.rs:21-39) compose to a permutation of at most
+//    2*(k1-k0) rows; one wave composes it, then every column stripe moves the
+//    affected rows in one read-all/write-all pass.  The same map updates the
+//    running row permutation perm (perm[i] = original row now at row i,
+//    src/decomposition/lu.rs:28-39).
+//  * trsm_llu: B := L^-1 B for a unit lower L (src/blas/trsm.rs:6-22) with the
+//    B stripe in LDS, 16-row diagonal sub-blocks solved per column and the rows
+//    below updated in parallel.
+//  * trtri_diag: inverses of the diagonal blocks of L (unit) and U.
+//  * trsv_dataflow: block-row workgroups that wait on per-block flags; tiles
+//    are fetched before waiting, so the sequential chain only carries one
+//    flag round trip, one tile product and one diagonal-inverse product per
+//    block (the substitutions of src/lapack/getrs.rs:24-36, blocked).
+#include <climits>
+
+#include "lair_internal.h"
+
+namespace lair {
+
+template <typename T>
+__device__ __forceinline__ void st_sc1(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T ld_sc1(const T *p) {
+  return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+constexpr int kSolveSpinLimit = 1 << 24;
+
+// Allows a kernel to use up to the full 160 KB of LDS per workgroup.
+// The attribute call's status is consumed here so that it cannot surface as
+// the error of a later launch.
+template <typename F>
+static void allow_large_lds(F *kernel) {
+  (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  (void)hipGetLastError();
+}
+
+// ------------------------------------------------------------- rowmap ---
+
+// rows[e] <- contents of src[e] (before the interchanges), e < *count.
+__global__ void __launch_bounds__(64)
+    rowmap_kernel(const int64_t *ipiv, int k0, int k1, int *rows, int *src, int *count) {
+  __shared__ int s_rows[kMaxSwapRows], s_src[kMaxSwapRows], s_piv[kMaxSwapRows / 2];
+  __shared__ int s_nx;
+  const int lane = threadIdx.x;
+  const int kb = k1 - k0;
+  for (int e = lane; e < kb; e += 64) {
+    s_rows[e] = k0 + e;
+    s_src[e] = k0 + e;
+    s_piv[e] = (int)ipiv[k0 + e];
+  }
+  if (lane == 0) s_nx = 0;
+  __syncthreads();
+  for (int e = 0; e < kb; ++e) {
+    const int k = k0 + e;
+    const int p = s_piv[e];
+    if (p == k) continue;
+    int ep;
+    if (p >= k0 && p < k1) {
+      ep = p - k0;
+    } else {
+      const int nx = s_nx;
+      ep = -1;
+      for (int b = 0; b < nx && ep < 0; b += 64) {
+        const bool hit = (b + lane < nx) && s_rows[kb + b + lane] == p;
+        const unsigned long long m = __ballot(hit);
+        if (m) ep = kb + b + __ffsll((long long)m) - 1;
+      }
+      if (ep < 0) {
+        ep = kb + nx;
+        if (lane == 0) {
+          s_rows[ep] = p;
+          s_src[ep] = p;
+          s_nx = nx + 1;
+        }
+      }
+      __syncthreads();
+    }
+    if (lane == 0) {
+      const int tmp = s_src[e];
+      s_src[e] = s_src[ep];
+      s_src[ep] = tmp;
+    }
+    __syncthreads();
+  }
+  const int total = kb + s_nx;
+  for (int e = lane; e < total; e += 64) {
+    rows[e] = s_rows[e];
+    src[e] = s_src[e];
+  }
+  if (lane == 0) *count = total;
+}
+
+// Columns [c0, c0+n1) and [s1, s1 + total-n1): 32 per workgroup.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    rowgather_kernel(T *a, int64_t lda, const int *rows, const int *src, const int *count, int c0,
+                     int n1, int s1, int total) {
+  extern __shared__ unsigned char smem_raw[];
+  T *buf = reinterpret_cast<T *>(smem_raw);  // [nr][32]
+  const int t = threadIdx.x, cl = t & 31, rg = t >> 5;
+  const int i = blockIdx.x * 32 + cl;
+  const bool act = i < total;
+  const int c = act ? ((i < n1) ? c0 + i : s1 + (i - n1)) : 0;
+  const int nr = *count;
+  for (int e = rg; e < nr; e += 8) {
+    const int s = src[e];
+    if (act && s != rows[e]) buf[e * 32 + cl] = a[(int64_t)s * lda + c];
+  }
+  __syncthreads();
+  for (int e = rg; e < nr; e += 8) {
+    const int s = src[e];
+    if (act && s != rows[e]) a[(int64_t)rows[e] * lda + c] = buf[e * 32 + cl];
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    permgather_kernel(int *perm, const int *rows, const int *src, const int *count) {
+  __shared__ int tmp[kMaxSwapRows];
+  const int nr = *count;
+  for (int e = threadIdx.x; e < nr; e += 256) tmp[e] = perm[src[e]];
+  __syncthreads();
+  for (int e = threadIdx.x; e < nr; e += 256) perm[rows[e]] = tmp[e];
+}
+
+template <typename T>
+hipError_t launch_rowswap(lair_context *ctx, T *a, int64_t lda, int c0, int c1, int s0, int s1,
+                          const int64_t *ipiv, int k0, int k1, int *perm) {
+  if (k1 <= k0) return hipSuccess;
+  if (k1 - k0 > kMaxSwapRows / 2) return hipErrorInvalidValue;
+  if (s0 < c0) s0 = c0;
+  if (s1 > c1) s1 = c1;
+  if (s1 < s0) s1 = s0;
+  const int n1 = s0 - c0;
+  const int total = n1 + (c1 - s1);
+  const int slot = prof_begin(ctx, LAIR_KCLASS_LASWP);
+  rowmap_kernel<<<1, 64, 0, ctx->stream>>>(ipiv, k0, k1, ctx->swap_rows, ctx->swap_src,
+                                           ctx->swap_count);
+  if (total > 0) {
+    const size_t shm = (size_t)2 * (k1 - k0) * 32 * sizeof(T);
+    static bool once = (allow_large_lds(rowgather_kernel<T>), true);
+    (void)once;
+    rowgather_kernel<T><<<(total + 31) / 32, 256, shm, ctx->stream>>>(
+        a, lda, ctx->swap_rows, ctx->swap_src, ctx->swap_count, c0, n1, s1, total);
+  }
+  if (perm) permgather_kernel<<<1, 256, 0, ctx->stream>>>(perm, ctx->swap_rows, ctx->swap_src,
+                                                          ctx->swap_count);
+  prof_end(ctx, slot, LAIR_KCLASS_LASWP, 0.0, 4.0 * sizeof(T) * (double)(k1 - k0) * total);
+  return hipGetLastError();
+}
+
+template hipError_t launch_rowswap<double>(lair_context *, double *, int64_t, int, int, int, int,
+                                           const int64_t *, int, int, int *);
+template hipError_t launch_rowswap<float>(lair_context *, float *, int64_t, int, int, int, int,
+                                          const int64_t *, int, int, int *);
+
+// perm from the full pivot vector (for factors supplied by the caller):
+// one thread applies the n interchanges to an LDS index array.
+__global__ void __launch_bounds__(256) perm_from_ipiv_kernel(const int64_t *ipiv, int n, int *perm) {
+  extern __shared__ int sp[];  // [n] when it fits, else unused
+  const bool lds = n <= kPermLdsMax;
+  int *p = lds ? sp : perm;
+  for (int i = threadIdx.x; i < n; i += 256) p[i] = i;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < n; ++i) {
+      const int q = (int)ipiv[i];
+      if (q != i) {
+        const int tmp = p[i];
+        p[i] = p[q];
+        p[q] = tmp;
+      }
+    }
+  }
+  __syncthreads();
+  if (lds)
+    for (int i = threadIdx.x; i < n; i += 256) perm[i] = p[i];
+}
+
+hipError_t launch_perm_from_ipiv(lair_context *ctx, const int64_t *ipiv, int n, int *perm) {
+  if (n <= 0) return hipSuccess;
+  const size_t shm = n <= kPermLdsMax ? (size_t)n * sizeof(int) : 0;
+  static bool once = (allow_large_lds(perm_from_ipiv_kernel), true);
+  (void)once;
+  perm_from_ipiv_kernel<<<1, 256, shm, ctx->stream>>>(ipiv, n, perm);
+  return hipGetLastError();
+}
+
+__global__ void iota_kernel(int *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = i;
+}
+
+hipError_t launch_iota(lair_context *ctx, int *p, int n) {
+  if (n <= 0) return hipSuccess;
+  iota_kernel<<<(n + 255) / 256, 256, 0, ctx->stream>>>(p, n);
+  return hipGetLastError();
+}
+
+// y[i][q] = (D) x[perm[i]][q]  (gather) or y[perm[i]][q] = (D) x[i][q] (scatter);
+// a null perm copies.
+template <typename S, typename D>
+__global__ void permute_rows_kernel(int rows, int cols, const int *perm, const S *x, int64_t ldx,
+                                    D *y, int64_t ldy, int scatter) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)rows * cols) return;
+  const int i = (int)(idx / cols), q = (int)(idx % cols);
+  const int pi = perm ? perm[i] : i;
+  if (!scatter)
+    y[(int64_t)i * ldy + q] = (D)x[(int64_t)pi * ldx + q];
+  else
+    y[(int64_t)pi * ldy + q] = (D)x[(int64_t)i * ldx + q];
+}
+
+template <typename S, typename D>
+hipError_t launch_permute_rows(lair_context *ctx, int rows, int cols, const int *perm, const S *x,
+                               int64_t ldx, D *y, int64_t ldy, bool scatter) {
+  const int64_t total = (int64_t)rows * cols;
+  if (total <= 0) return hipSuccess;
+  permute_rows_kernel<S, D><<<(unsigned)((total + 255) / 256), 256, 0, ctx->stream>>>(
+      rows, cols, perm, x, ldx, y, ldy, scatter ? 1 : 0);
+  return hipGetLastError();
+}
+
+template hipError_t launch_permute_rows<double, double>(lair_context *, int, int, const int *,
+                                                        const double *, int64_t, double *, int64_t,
+                                                        bool);
+template hipError_t launch_permute_rows<double, float>(lair_context *, int, int, const int *,
+                                                       const double *, int64_t, float *, int64_t,
+                                                       bool);
+template hipError_t launch_permute_rows<float, float>(lair_context *, int, int, const int *,
+                                                      const float *, int64_t, float *, int64_t,
+                                                      bool);
+
+// ------------------------------------------------------------ trsm_llu ---
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    trsm_llu_kernel(int kb, int ncols, const T *__restrict__ L, int64_t ldl, T *__restrict__ B,
+                    int64_t ldb) {
+  extern __shared__ unsigned char smem_raw[];
+  T *xs = reinterpret_cast<T *>(smem_raw);  // [kb][64]
+  const int t = threadIdx.x, col = t & 63, rg = t >> 6;
+  const int c = blockIdx.x * 64 + col;
+  const bool act = c < ncols;
+  for (int i = rg; i < kb; i += 4) xs[i * 64 + col] = act ? B[(int64_t)i * ldb + c] : T(0);
+  __syncthreads();
+  for (int s0 = 0; s0 < kb; s0 += 16) {
+    const int sb = kb - s0 < 16 ? kb - s0 : 16;
+    if (rg == 0) {
+      for (int i = 1; i < sb; ++i) {
+        T acc = xs[(s0 + i) * 64 + col];
+        const T *lr = L + (int64_t)(s0 + i) * ldl + s0;
+        for (int p = 0; p < i; ++p) acc -= lr[p] * xs[(s0 + p) * 64 + col];
+        xs[(s0 + i) * 64 + col] = acc;
+      }
+    }
+    __syncthreads();
+    for (int r = s0 + sb + rg; r < kb; r += 4) {
+      T acc = xs[r * 64 + col];
+      const T *lr = L + (int64_t)r * ldl + s0;
+#pragma unroll 4
+      for (int p = 0; p < sb; ++p) acc -= lr[p] * xs[(s0 + p) * 64 + col];
+      xs[r * 64 + col] = acc;
+    }
+    __syncthreads();
+  }
+  if (act)
+    for (int i = rg; i < kb; i += 4) B[(int64_t)i * ldb + c] = xs[i * 64 + col];
+}
+
+template <typename T>
+hipError_t launch_trsm_llu(lair_context *ctx, int kb, int ncols, const T *L, int64_t ldl, T *B,
+                           int64_t ldb) {
+  if (kb <= 0 || ncols <= 0) return hipSuccess;
+  const size_t shm = (size_t)kb * 64 * sizeof(T);
+  if (shm > 160 * 1024) return hipErrorInvalidValue;
+  static bool once = (allow_large_lds(trsm_llu_kernel<T>), true);
+  (void)once;
+  const int slot = prof_begin(ctx, LAIR_KCLASS_TRSM);
+  trsm_llu_kernel<T><<<(ncols + 63) / 64, 256, shm, ctx->stream>>>(kb, ncols, L, ldl, B, ldb);
+  prof_end(ctx, slot, LAIR_KCLASS_TRSM, (double)kb * kb * ncols,
+           sizeof(T) * (2.0 * kb * ncols + 0.5 * kb * kb));
+  return hipGetLastError();
+}
+
+template hipError_t launch_trsm_llu<double>(lair_context *, int, int, const double *, int64_t,
+                                            double *, int64_t);
+template hipError_t launch_trsm_llu<float>(lair_context *, int, int, const float *, int64_t, float *,
+                                           int64_t);
+
+// ---------------------------------------------------------- trtri_diag ---
+
+// blockIdx.x = diagonal block; blockIdx.y = 0: unit lower L -> linv,
+// 1: upper U -> uinv.  Each inverse is stored dense BS x BS (row-major).
+template <typename T, int BS>
+__global__ void __launch_bounds__(BS)
+    trtri_diag_kernel(const T *__restrict__ lu, int64_t lda, int n, T *__restrict__ linv,
+                      T *__restrict__ uinv) {
+  extern __shared__ unsigned char smem_raw[];
+  T *M = reinterpret_cast<T *>(smem_raw);  // [BS][BS+1]
+  T *X = M + BS * (BS + 1);                // [BS][BS+1]
+  constexpr int LD = BS + 1;
+  const int b = blockIdx.x, j = threadIdx.x;
+  const bool upper = blockIdx.y == 1;
+  const int r0 = b * BS;
+  const int nb = n - r0 < BS ? n - r0 : BS;
+  for (int e = j; e < BS * BS; e += BS) {
+    const int r = e / BS, c = e % BS;
+    M[r * LD + c] = (r < nb && c < nb) ? lu[(int64_t)(r0 + r) * lda + r0 + c] : T(0);
+    X[r * LD + c] = T(0);
+  }
+  __syncthreads();
+  if (j < nb) {
+    if (!upper) {
+      X[j * LD + j] = T(1);
+      for (int i = j + 1; i < nb; ++i) {
+        T a0 = T(0), a1 = T(0);
+        int p = j;
+        for (; p + 1 < i; p += 2) {
+          a0 -= M[i * LD + p] * X[p * LD + j];
+          a1 -= M[i * LD + p + 1] * X[(p + 1) * LD + j];
+        }
+        if (p < i) a0 -= M[i * LD + p] * X[p * LD + j];
+        X[i * LD + j] = a0 + a1;
+      }
+    } else {
+      X[j * LD + j] = T(1) / M[j * LD + j];
+      for (int i = j - 1; i >= 0; --i) {
+        T a0 = T(0), a1 = T(0);
+        int p = i + 1;
+        for (; p + 1 <= j; p += 2) {
+          a0 -= M[i * LD + p] * X[p * LD + j];
+          a1 -= M[i * LD + p + 1] * X[(p + 1) * LD + j];
+        }
+        if (p <= j) a0 -= M[i * LD + p] * X[p * LD + j];
+        X[i * LD + j] = (a0 + a1) / M[i * LD + i];
+      }
+    }
+  }
+  __syncthreads();
+  T *out = (upper ? uinv : linv) + (size_t)b * BS * BS;
+  for (int e = j; e < BS * BS; e += BS) out[e] = X[(e / BS) * LD + e % BS];
+}
+
+template <typename T>
+hipError_t launch_trtri_diag(lair_context *ctx, const T *lu, int64_t lda, int n, T *linv, T *uinv) {
+  if (n <= 0) return hipSuccess;
+  constexpr int BS = TrsvBlock<T>::BS;
+  const int nblk = (n + BS - 1) / BS;
+  const size_t shm = 2 * (size_t)BS * (BS + 1) * sizeof(T);
+  static bool once = (allow_large_lds(trtri_diag_kernel<T, BS>), true);
+  (void)once;
+  const int slot = prof_begin(ctx, LAIR_KCLASS_GETRS);
+  trtri_diag_kernel<T, BS><<<dim3(nblk, 2), BS, shm, ctx->stream>>>(lu, lda, n, linv, uinv);
+  prof_end(ctx, slot, LAIR_KCLASS_GETRS, (double)nblk * BS * BS * BS / 3.0);
+  return hipGetLastError();
+}
+
+template hipError_t launch_trtri_diag<double>(lair_context *, const double *, int64_t, int, double *,
+                                              double *);
+template hipError_t launch_trtri_diag<float>(lair_context *, const float *, int64_t, int, float *,
+                                             float *);
+
+// ------------------------------------------------------ trsv_dataflow ---
+
+template <typename T>
+struct TrsvArgs {
+  const T *lu;
+  int64_t lda;
+  const T *dinv;  // inverses of the diagonal blocks used by this solve
+  const T *xin;   // right-hand sides (may alias x)
+  T *x;           // n x nrhs solution
+  int64_t ldx;
+  int n, nrhs, nblk, nchunks;
+  uint32_t *flags;  // [nblk][nchunks]
+  uint32_t epoch;
+  int forward, trans, dinv_trans;
+  int32_t *err;
+};
+
+// Effective matrix E: forward & !trans -> L (strictly lower part of lu);
+// !forward & !trans -> U; forward & trans -> U^T; !forward & trans -> L^T.
+// Workgroup (block row i, column chunk ch) computes
+//   x_i = D_i^-1 (x_i - sum_j E_ij x_j)
+// over the blocks j already solved (j < i forward, j > i backward).  Tiles
+// move global -> registers (16-byte lane-contiguous loads, issued before the
+// wait for the previous block) -> LDS -> row segments per thread.
+template <typename T, int BS, int RC>
+__global__ void __launch_bounds__(256) trsv_dataflow_kernel(TrsvArgs<T> p) {
+  constexpr int TPR = 256 / BS;   // threads per row in the product
+  constexpr int SEG = BS / TPR;   // columns per thread
+  constexpr int VEC = 16 / sizeof(T);           // elements per 16-byte load
+  constexpr int NV = BS * BS / VEC / 256;       // 16-byte loads per thread per tile
+  constexpr int LDT = BS + VEC;                 // padded LDS row
+  typedef T vec_t __attribute__((ext_vector_type(VEC)));
+  extern __shared__ unsigned char smem_raw[];
+  T *tile = reinterpret_cast<T *>(smem_raw);    // [BS][LDT]
+  __shared__ T xs[BS][RC];
+  __shared__ T ys[BS][RC];
+  __shared__ int s_bail;
+  const int t = threadIdx.x;
+  const int r = t / TPR, h = t % TPR;
+  const int wid = blockIdx.x;
+  const int ib = wid / p.nchunks, ch = wid % p.nchunks;
+  const int i = p.forward ? ib : p.nblk - 1 - ib;
+  const int r0 = i * BS;
+  const int c0 = ch * RC;
+  const int ncols = p.nrhs - c0 < RC ? p.nrhs - c0 : RC;
+  const int gr = r0 + r;
+  const bool rowok = gr < p.n;
+
+  // Register staging of one BS x BS block of a row-major matrix (base, ld) at
+  // block coordinates (rb, cb), optionally transposed; out-of-range is zero.
+  // Element e of load k covers row (t*VEC + k*256*VEC) / BS.
+  auto fetch = [&](const T *base, int64_t ld, int rb, int cb, vec_t (&v)[NV]) {
+    const bool aligned = ((ld * sizeof(T)) % 16 == 0) && (reinterpret_cast<uintptr_t>(base) % 16 == 0);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const int lin = (k * 256 + t) * VEC;
+      const int rr = lin / BS, cc = lin % BS;
+      const int grow = rb + rr, gcol = cb + cc;
+      if (aligned && grow < p.n && gcol + VEC <= p.n) {
+        v[k] = *reinterpret_cast<const vec_t *>(base + (int64_t)grow * ld + gcol);
+      } else {
+#pragma unroll
+        for (int q = 0; q < VEC; ++q)
+          v[k][q] = (grow < p.n && gcol + q < p.n) ? base[(int64_t)grow * ld + gcol + q] : T(0);
+      }
+    }
+  };
+  // Stores the staged block into LDS as E (row r, column c), transposing when
+  // the staged block is stored M[c][r].
+  auto stash = [&](const vec_t (&v)[NV], bool transposed) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const int lin = (k * 256 + t) * VEC;
+      const int rr = lin / BS, cc = lin % BS;
+      if (!transposed) {
+        *reinterpret_cast<vec_t *>(tile + rr * LDT + cc) = v[k];
+      } else {
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) tile[(cc + q) * LDT + rr] = v[k][q];
+      }
+    }
+  };
+
+  // Diagonal inverse D_i (dense BS x BS, row-major): row segment per thread.
+  T dreg[SEG];
+  {
+    vec_t v[NV];
+    const T *D = p.dinv + (size_t)i * BS * BS;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const int lin = (k * 256 + t) * VEC;
+      v[k] = *reinterpret_cast<const vec_t *>(D + lin);
+    }
+    stash(v, p.dinv_trans != 0);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < SEG; ++s) dreg[s] = tile[r * LDT + h * SEG + s];
+    __syncthreads();
+  }
+  T acc[RC];
+#pragma unroll
+  for (int q = 0; q < RC; ++q) acc[q] = T(0);
+  if (t == 0) s_bail = 0;
+
+  const int nb = p.forward ? i : p.nblk - 1 - i;  // number of dependency blocks
+  auto issue = [&](int d, vec_t (&v)[NV]) {
+    const int j = p.forward ? d : p.nblk - 1 - d;
+    if (!p.trans)
+      fetch(p.lu, p.lda, r0, j * BS, v);   // E_ij = M[r0.., jBS..]
+    else
+      fetch(p.lu, p.lda, j * BS, r0, v);   // E_ij = (M[jBS.., r0..])^T
+  };
+  vec_t stage[NV];
+  if (nb > 0) issue(0, stage);
+  for (int d = 0; d < nb; ++d) {
+    const int j = p.forward ? d : p.nblk - 1 - d;
+    const int cb = j * BS;
+    stash(stage, p.trans != 0);
+    if (d + 1 < nb) issue(d + 1, stage);
+    if (t == 0) {
+      const uint32_t *f = p.flags + (size_t)j * p.nchunks + ch;
+      int spins = 0;
+      while ((int32_t)(ld_sc1(f) - p.epoch) < 0) {
+        if (++spins > kSolveSpinLimit || ld_sc1(p.err) != 0) {
+          st_sc1(p.err, 1);
+          s_bail = 1;
+          break;
+        }
+      }
+    }
+    __syncthreads();
+    if (s_bail) return;
+    for (int e = t; e < BS * RC; e += 256) {
+      const int rr = e / RC, q = e % RC;
+      const int gi = cb + rr;
+      xs[rr][q] = (gi < p.n && q < ncols) ? ld_sc1(p.x + (int64_t)gi * p.ldx + c0 + q) : T(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < SEG; ++s) {
+      const int c = h * SEG + s;
+      const T e = tile[r * LDT + c];
+#pragma unroll
+      for (int q = 0; q < RC; ++q) acc[q] += e * xs[c][q];
+    }
+    __syncthreads();
+  }
+  // Combine the TPR partial sums of each row (consecutive lanes).
+#pragma unroll
+  for (int q = 0; q < RC; ++q) {
+#pragma unroll
+    for (int off = TPR / 2; off >= 1; off >>= 1) acc[q] += __shfl_xor(acc[q], off);
+  }
+  if (h == 0) {
+#pragma unroll
+    for (int q = 0; q < RC; ++q) {
+      T bval = (rowok && q < ncols) ? p.xin[(int64_t)gr * p.ldx + c0 + q] : T(0);
+      ys[r][q] = bval - acc[q];
+    }
+  }
+  __syncthreads();
+  T out[RC];
+#pragma unroll
+  for (int q = 0; q < RC; ++q) out[q] = T(0);
+#pragma unroll
+  for (int s = 0; s < SEG; ++s) {
+    const int c = h * SEG + s;
+#pragma unroll
+    for (int q = 0; q < RC; ++q) out[q] += dreg[s] * ys[c][q];
+  }
+#pragma unroll
+  for (int q = 0; q < RC; ++q) {
+#pragma unroll
+    for (int off = TPR / 2; off >= 1; off >>= 1) out[q] += __shfl_xor(out[q], off);
+  }
+  if (h == 0 && rowok) {
+#pragma unroll
+    for (int q = 0; q < RC; ++q)
+      if (q < ncols) st_sc1(p.x + (int64_t)gr * p.ldx + c0 + q, out[q]);
+  }
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  if (t == 0) st_sc1(p.flags + (size_t)i * p.nchunks + ch, p.epoch);
+}
+
+template <typename T>
+hipError_t launch_trsv_dataflow(lair_context *ctx, const T *lu, int64_t lda, const T *dinv,
+                                bool dinv_trans, const T *xin, T *x, int64_t ldx, int n, int nrhs,
+                                bool forward, bool trans) {
+  if (n <= 0 || nrhs <= 0) return hipSuccess;
+  constexpr int BS = TrsvBlock<T>::BS;
+  const int rc = nrhs == 1 ? 1 : (nrhs <= 4 ? 4 : 16);
+  const int nblk = (n + BS - 1) / BS;
+  const int nchunks = (nrhs + rc - 1) / rc;
+  hipError_t e = ensure(ctx->ws_flags, sizeof(uint32_t) * (size_t)nblk * nchunks);
+  if (e != hipSuccess) return e;
+  if (ctx->ws_flags_zeroed < ctx->ws_flags.bytes || ctx->tri_epoch > 0x70000000u) {
+    e = launch_fill_u32(ctx, static_cast<uint32_t *>(ctx->ws_flags.ptr),
+                        (int64_t)(ctx->ws_flags.bytes / sizeof(uint32_t)), 0u);
+    if (e != hipSuccess) return e;
+    ctx->ws_flags_zeroed = ctx->ws_flags.bytes;
+    ctx->tri_epoch = 0;
+  }
+  TrsvArgs<T> a;
+  a.lu = lu;
+  a.lda = lda;
+  a.dinv = dinv;
+  a.xin = xin;
+  a.x = x;
+  a.ldx = ldx;
+  a.n = n;
+  a.nrhs = nrhs;
+  a.nblk = nblk;
+  a.nchunks = nchunks;
+  a.flags = static_cast<uint32_t *>(ctx->ws_flags.ptr);
+  a.epoch = ++ctx->tri_epoch;
+  a.forward = forward ? 1 : 0;
+  a.trans = trans ? 1 : 0;
+  a.dinv_trans = dinv_trans ? 1 : 0;
+  a.err = ctx->err;
+  const unsigned grid = (unsigned)(nblk * nchunks);
+  const size_t shm = (size_t)BS * (BS + 16 / sizeof(T)) * sizeof(T);
+  static bool once = (allow_large_lds(trsv_dataflow_kernel<T, BS, 1>),
+                      allow_large_lds(trsv_dataflow_kernel<T, BS, 4>),
+                      allow_large_lds(trsv_dataflow_kernel<T, BS, 16>), true);
+  (void)once;
+  const int slot = prof_begin(ctx, LAIR_KCLASS_GETRS);
+  if (rc == 1)
+    trsv_dataflow_kernel<T, BS, 1><<<grid, 256, shm, ctx->stream>>>(a);
+  else if (rc == 4)
+    trsv_dataflow_kernel<T, BS, 4><<<grid, 256, shm, ctx->stream>>>(a);
+  else
+    trsv_dataflow_kernel<T, BS, 16><<<grid, 256, shm, ctx->stream>>>(a);
+  prof_end(ctx, slot, LAIR_KCLASS_GETRS, (double)n * n * nrhs,
+           sizeof(T) * (0.5 * (double)n * n + 2.0 * n * nrhs));
+  return hipGetLastError();
+}
+
+template hipError_t launch_trsv_dataflow<double>(lair_context *, const double *, int64_t,
+                                                 const double *, bool, const double *, double *,
+                                                 int64_t, int, int, bool, bool);
+template hipError_t launch_trsv_dataflow<float>(lair_context *, const float *, int64_t,
+                                                const float *, bool, const float *, float *, int64_t,
+                                                int, int, bool, bool);
+
+}  // namespace lair

@@ -195,6 +195,7 @@ def test_c2_fixture_n8192_device():
         np.testing.assert_array_equal(piv, np.array(fx["pivots"]))
         db = L.DeviceArray.from_host(b)
         L.check(ctx.lib.lair_dgetrs_device(ctx.h, n, nrhs, da.ptr, ld, dp.ptr, db.ptr, nrhs, 0), "getrs")
+        ctx.synchronize()
         x = db.download()
     assert solve_backward_error(a, x, b) <= n * EPS
 

@@ -100,14 +100,14 @@ def test_profiling_counts_kernels(ctx):
     da = L.DeviceArray.from_host(a)
     dp = L.DeviceArray(n, 1, np.int64)
     ctx.lib.lair_profile_reset(ctx.h)
-    ctx.lib.lair_profile_enable(ctx.h, 1)
+    ctx.lib.lair_profile_enable(ctx.h, 0xFF)
     sing = ctypes.c_int64(-1)
     L.check(ctx.lib.lair_dgetrf_device(ctx.h, n, n, da.ptr, n, dp.ptr, 128, ctypes.byref(sing)), "getrf")
     ctx.lib.lair_profile_enable(ctx.h, 0)
     ms, cnt, fl = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
     L.check(ctx.lib.lair_profile_read(ctx.h, _capi.KCLASS["gemm"], ctypes.byref(ms), ctypes.byref(cnt),
-                                      ctypes.byref(fl)), "read")
+                                      ctypes.byref(fl), None), "read")
     assert cnt.value > 0 and ms.value > 0 and fl.value > 0
     L.check(ctx.lib.lair_profile_read(ctx.h, _capi.KCLASS["panel"], ctypes.byref(ms), ctypes.byref(cnt),
-                                      ctypes.byref(fl)), "read")
+                                      ctypes.byref(fl), None), "read")
     assert cnt.value == n // 16
