@@ -159,6 +159,51 @@ lair_status lair_dbackward_error(lair_context *ctx, int64_t n, int64_t nrhs, con
                                  int64_t lda, const double *b, int64_t ldb, const double *x,
                                  int64_t ldx, double *berr);
 
+/* ------------------------------------------------------------ batched --- */
+
+/* Batched solve of `count` independent systems A_i X_i = B_i with
+ * 1 <= n_i <= 256 and 1 <= nrhs_i <= 16 (equation::solve, src/equation.rs:32-60,
+ * applied to a batch; no reference counterpart for the batching itself).
+ * Packed device layout, systems in order: A_i row-major n_i x n_i at element
+ * offset sum_{j<i} n_j^2 of `a`; B_i row-major n_i x nrhs_i at offset
+ * sum_{j<i} n_j nrhs_j of `b`; pivots of A_i (0-based, int64) at offset
+ * sum_{j<i} n_j of `ipiv`; info[i] = -1 or the last zero-pivot column.
+ * a is overwritten by the LU factors and b by X.  precision: 0 = FP64,
+ * 1 = FP32 (a, b are float).  The plan sorts the systems by decreasing cost,
+ * optionally autotunes the launch (flags & LAIR_BATCHED_AUTOTUNE: candidate
+ * shapes are timed on scratch copies of a and b) and captures the launch in a
+ * hipGraph on the context's stream. */
+#define LAIR_BATCHED_AUTOTUNE 1
+typedef struct lair_batched_plan lair_batched_plan;
+lair_status lair_gesv_batched_plan(lair_context *ctx, int32_t precision, int32_t count,
+                                   const int32_t *n, const int32_t *nrhs, void *a, void *b,
+                                   int64_t *ipiv, int32_t *info, int32_t flags,
+                                   lair_batched_plan **plan);
+/* Replays the captured graph (asynchronous on the context's stream). */
+lair_status lair_gesv_batched_run(lair_batched_plan *plan);
+/* Launch shape chosen by the plan: panel width, workgroups, LDS budget (elements). */
+lair_status lair_gesv_batched_config(lair_batched_plan *plan, int32_t *nb, int32_t *grid,
+                                     int32_t *lds_elems);
+lair_status lair_gesv_batched_destroy(lair_batched_plan *plan);
+
+/* ------------------------------------------------------------ adjoint --- */
+
+/* Adjoint of X = A^-1 B (no counterpart in the reference; the reverse-mode
+ * derivative of equation::solve, src/equation.rs:32-60).  From the factors
+ * (lu, ipiv) of A, the solution X (n x nrhs) and the upstream gradient XBAR:
+ *   BBAR = A^-T XBAR   (transposed getrs on the stored factors)
+ *   ABAR = -BBAR X^T   (n x n, FP64 MFMA GEMM, written without reading ABAR)
+ * All arrays are device-resident and row-major.  Synchronizes the stream. */
+lair_status lair_dsolve_backward_device(lair_context *ctx, int64_t n, int64_t nrhs,
+                                        const double *lu, int64_t lda, const int64_t *ipiv,
+                                        const double *x, int64_t ldx, const double *xbar,
+                                        int64_t ldxb, double *bbar, int64_t ldbb, double *abar,
+                                        int64_t ldab);
+/* Host-pointer form with contiguous row-major arrays (lu n x n, x/xbar/bbar
+ * n x nrhs, abar n x n). */
+lair_status lair_dsolve_backward(int64_t n, int64_t nrhs, const double *lu, const int64_t *ipiv,
+                                 const double *x, const double *xbar, double *bbar, double *abar);
+
 /* ----------------------------------------------- instrumentation ------- */
 
 /* Kernel classes timed with HIP events on the context's stream while

@@ -20,8 +20,8 @@ import numpy as np
 from . import _capi
 from ._capi import LairError, check
 
-__all__ = ["getrf", "getrs", "laswp", "InvalidInput", "ShapeError", "ValueError_", "equation", "lu",
-           "Context", "DeviceArray", "LairError"]
+__all__ = ["getrf", "getrs", "laswp", "solve_backward", "InvalidInput", "ShapeError", "ValueError_", "equation", "lu",
+           "Context", "DeviceArray", "BatchedPlan", "LairError"]
 
 
 class InvalidInput(Exception):
@@ -101,6 +101,60 @@ def laswp(ncols, a, begin, piv):
     rs, cs = _elem_strides(a)
     check(lib.lair_dlaswp(ncols, _ptr(a), rs, cs, begin, len(piv), piv.ctypes.data_as(_capi.P64)),
           "laswp")
+
+
+def solve_backward(lu_factors, pivots, x, xbar):
+    """Adjoint of X = A^-1 B from A's factors (getrf output), the solution X
+    and the upstream gradient XBAR: returns (BBAR, ABAR) with
+    BBAR = A^-T XBAR (transposed getrs reusing the factors and pivots) and
+    ABAR = -BBAR X^T (FP64 MFMA GEMM).  X and XBAR are n or n x nrhs."""
+    lib = _capi.load()
+    lu_c = np.ascontiguousarray(lu_factors, np.float64)
+    n = lu_c.shape[0]
+    piv = np.ascontiguousarray(pivots, np.int64)
+    x2 = np.ascontiguousarray(np.asarray(x, np.float64).reshape(n, -1))
+    xb2 = np.ascontiguousarray(np.asarray(xbar, np.float64).reshape(n, -1))
+    if x2.shape != xb2.shape or len(piv) != n or lu_c.shape != (n, n):
+        raise ValueError("solve_backward: shape mismatch")
+    bbar = np.zeros_like(x2)
+    abar = np.zeros((n, n), np.float64)
+    check(lib.lair_dsolve_backward(n, x2.shape[1], _ptr(lu_c), piv.ctypes.data_as(_capi.P64), _ptr(x2),
+                                   _ptr(xb2), _ptr(bbar), _ptr(abar)), "solve_backward")
+    return bbar.reshape(np.shape(xbar)), abar
+
+
+class BatchedPlan:
+    """Plan for solving many small systems A_i X_i = B_i in one persistent,
+    graph-captured launch (packed device layout of include/lair_dev.h).
+
+    ``a``, ``b``, ``ipiv`` and ``info`` are DeviceArray buffers holding the
+    packed systems; ``run()`` overwrites a with the LU factors and b with X."""
+
+    def __init__(self, ctx, ns, nrhs, a, b, ipiv, info, precision=0, autotune=True):
+        self.ctx = ctx
+        lib = ctx.lib
+        ns = np.ascontiguousarray(ns, np.int32)
+        nrhs = np.ascontiguousarray(nrhs, np.int32)
+        h = ctypes.c_void_p()
+        P32 = ctypes.POINTER(ctypes.c_int32)
+        check(lib.lair_gesv_batched_plan(ctx.h, precision, len(ns), ns.ctypes.data_as(P32),
+                                         nrhs.ctypes.data_as(P32), a.ptr, b.ptr, ipiv.ptr, info.ptr,
+                                         1 if autotune else 0, ctypes.byref(h)), "batched_plan")
+        self.h = h
+
+    def run(self):
+        check(self.ctx.lib.lair_gesv_batched_run(self.h), "batched_run")
+
+    def config(self):
+        nb, grid, lds = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        check(self.ctx.lib.lair_gesv_batched_config(self.h, ctypes.byref(nb), ctypes.byref(grid),
+                                                    ctypes.byref(lds)), "batched_config")
+        return {"nb": nb.value, "grid": grid.value, "lds_elems": lds.value}
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.lair_gesv_batched_destroy(self.h)
+            self.h = None
 
 
 class equation:

@@ -71,6 +71,14 @@ SIGNATURES = [
      [vp, i64, i64, vp, i64, vp, i64, vp, i64, i32, i32, f64, ctypes.POINTER(IrInfo)]),
     ("lair_dgesv_device", i32, [vp, i64, i64, vp, i64, vp, i64, vp, i64]),
     ("lair_dbackward_error", i32, [vp, i64, i64, vp, i64, vp, i64, vp, i64, PD]),
+    ("lair_dsolve_backward_device", i32,
+     [vp, i64, i64, vp, i64, vp, vp, i64, vp, i64, vp, i64, vp, i64]),
+    ("lair_dsolve_backward", i32, [i64, i64, vp, P64, vp, vp, vp, vp]),
+    ("lair_gesv_batched_plan", i32,
+     [vp, i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), vp, vp, vp, vp, i32, ctypes.POINTER(vp)]),
+    ("lair_gesv_batched_run", i32, [vp]),
+    ("lair_gesv_batched_config", i32, [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]),
+    ("lair_gesv_batched_destroy", i32, [vp]),
     ("lair_profile_enable", i32, [vp, i32]),
     ("lair_profile_read", i32, [vp, i32, PD, P64, PD, PD]),
     ("lair_profile_reset", i32, [vp]),

@@ -1,0 +1,473 @@
+// Batched solve of many independent small systems A_i X_i = B_i (n_i <= 256,
+// nrhs_i <= 16) in one persistent launch.
+//
+// Workgroups pull system indices from an atomic queue (an order sorted by
+// decreasing n^3, so the largest systems start first) until the queue is
+// empty; every workgroup reaches that exit.  Each system is factored as the
+// augmented matrix [A | B] with a blocked right-looking LU with partial
+// pivoting (src/lapack/getrf.rs:46-120, blocked; the interchanges are applied
+// to B as getrs.rs:23 does), so the forward substitution comes with the
+// factorization; a blocked back substitution with U gives X (getrs.rs:30-36).
+//
+// Per panel of NB columns:
+//  * wave 0 keeps the panel rows in registers (lane l: rows l, l+64, ...) and
+//    does, per column, a DPP argmax (ties -> lowest row; NaN and zero never
+//    chosen, src/blas/iamax.rs:6-21), the interchange, the reciprocal scaling
+//    and the rank-1 update (getrf.rs:76-88);
+//  * the panel's interchanges compose into one row map, applied to the other
+//    columns of [A | B] as a gather;
+//  * U12 = L11^-1 A12 column by column, then A22 -= L21 U12 with L21 and U12
+//    staged in LDS and 4 x 4 register tiles.
+// The augmented matrix lives in LDS when it fits (copied in and out), else it
+// is updated in place in global memory (A and B buffers).
+#include <algorithm>
+#include <climits>
+#include <numeric>
+#include <vector>
+
+#include "lair_internal.h"
+
+namespace lair {
+
+constexpr int kBatchMaxN = 256;
+constexpr int kBatchMaxRhs = 16;
+constexpr int kBatchLDU = kBatchMaxN + kBatchMaxRhs + 1;
+
+template <typename T>
+struct BatchArgs {
+  int count;
+  const int *n;          // [count]
+  const int *nrhs;       // [count]
+  const int64_t *a_off;  // element offsets of A_i (row-major, ld = n_i)
+  const int64_t *b_off;  // element offsets of B_i (row-major, ld = nrhs_i)
+  const int64_t *p_off;  // offsets of ipiv_i (n_i entries)
+  const int *order;      // processing order
+  T *a;
+  T *b;
+  int64_t *ipiv;
+  int *info;             // -1, or the last zero-pivot column (getrf.rs:72-73)
+  int *queue;            // work counter, zero at launch
+  int lds_max_elems;     // largest augmented matrix (n * (n + nrhs)) kept in LDS
+};
+
+// [A | B] with A and B in separate row-major buffers.
+template <typename T>
+struct SplitMat {
+  T *a;
+  T *b;
+  int n, r;
+  __device__ __forceinline__ T &operator()(int i, int c) const {
+    return c < n ? a[i * n + c] : b[i * r + (c - n)];
+  }
+};
+
+// [A | B] contiguous (leading dimension n + r), e.g. in LDS.
+template <typename T>
+struct DenseMat {
+  T *m;
+  int ld;
+  __device__ __forceinline__ T &operator()(int i, int c) const { return m[i * ld + c]; }
+};
+
+template <int CTRL, int ROWM = 0xF>
+__device__ __forceinline__ double bdpp(double x) {
+  const int lo = __double2loint(x), hi = __double2hiint(x);
+  const int l2 = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROWM, 0xF, false);
+  const int h2 = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROWM, 0xF, false);
+  return __hiloint2double(h2, l2);
+}
+
+// Maximum of non-negative doubles over the wave (result uniform).
+__device__ __forceinline__ double bwave_max(double v) {
+  v = fmax(v, bdpp<0xB1>(v));        // quad_perm [1,0,3,2]
+  v = fmax(v, bdpp<0x4E>(v));        // quad_perm [2,3,0,1]
+  v = fmax(v, bdpp<0x141>(v));       // row_half_mirror
+  v = fmax(v, bdpp<0x140>(v));       // row_mirror
+  v = fmax(v, bdpp<0x142, 0xA>(v));  // row_bcast:15
+  v = fmax(v, bdpp<0x143, 0xC>(v));  // row_bcast:31
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T, int NB>
+struct BatchSolver {
+  static constexpr int RPT = kBatchMaxN / 64;  // panel rows per lane
+  int t, lane, wave;
+  T *sL;      // [kBatchMaxN][NB]: L11 and L21 of the current panel
+  T *sU;      // [NB][kBatchLDU]: U12 (and the rows of B) of the current panel
+  T *sprow;   // [2][NB]: pivot row and diagonal row
+  int *spiv;  // [NB]
+  int *srows, *ssrc, *scnt;  // row map of the panel's interchanges
+  int *sinfo;
+
+  // Row map of the interchanges spiv[0..w) (absolute rows >= k0): rows[e]
+  // receives the contents of src[e].  Lane 0 of wave 0 composes it.
+  __device__ void compose_rowmap(int k0, int w) {
+    if (lane != 0) return;
+    int cnt = w;
+    for (int e = 0; e < w; ++e) {
+      srows[e] = k0 + e;
+      ssrc[e] = k0 + e;
+    }
+    for (int e = 0; e < w; ++e) {
+      const int p = spiv[e];
+      if (p == k0 + e) continue;
+      int ep = -1;
+      if (p < k0 + w) {
+        ep = p - k0;
+      } else {
+        for (int q = w; q < cnt; ++q)
+          if (srows[q] == p) {
+            ep = q;
+            break;
+          }
+        if (ep < 0) {
+          ep = cnt++;
+          srows[ep] = p;
+          ssrc[ep] = p;
+        }
+      }
+      const int tmp = ssrc[e];
+      ssrc[e] = ssrc[ep];
+      ssrc[ep] = tmp;
+    }
+    *scnt = cnt;
+  }
+
+  template <class Mat>
+  __device__ void panel(const Mat &M, int n, int kb, int w, int64_t *ipiv_out) {
+#pragma clang fp contract(off)
+    T v[RPT][NB];
+#pragma unroll
+    for (int s = 0; s < RPT; ++s) {
+      const int row = kb + lane + 64 * s;
+#pragma unroll
+      for (int c = 0; c < NB; ++c) v[s][c] = (row < n && c < w) ? M(row, kb + c) : T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      if (j < w) {
+        const int d = kb + j;
+        double bv = 0.0;
+        int br = INT_MAX;
+#pragma unroll
+        for (int s = 0; s < RPT; ++s) {
+          const int row = kb + lane + 64 * s;
+          if (row >= d && row < n) {
+            const double x = fabs((double)v[s][j]);
+            if (x > bv) {
+              bv = x;
+              br = row;
+            }
+          }
+        }
+        const double vmax = bwave_max(bv);
+        const unsigned long long m = __ballot(br != INT_MAX && bv == vmax);
+        int pr = INT_MAX;
+        unsigned long long mm = m;
+        while (mm) {
+          const int l = __ffsll((long long)mm) - 1;
+          mm &= mm - 1;
+          const int rl = __builtin_amdgcn_readlane(br, l);
+          pr = rl < pr ? rl : pr;
+        }
+        const bool singular = (pr == INT_MAX);
+        if (singular) pr = d;
+#pragma unroll
+        for (int s = 0; s < RPT; ++s) {
+          const int row = kb + lane + 64 * s;
+          if (row == pr) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) sprow[c] = v[s][c];
+          }
+          if (row == d) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) sprow[NB + c] = v[s][c];
+          }
+        }
+        wave_sync();
+        if (pr != d) {
+#pragma unroll
+          for (int s = 0; s < RPT; ++s) {
+            const int row = kb + lane + 64 * s;
+            if (row == d) {
+#pragma unroll
+              for (int c = 0; c < NB; ++c) v[s][c] = sprow[c];
+            } else if (row == pr) {
+#pragma unroll
+              for (int c = 0; c < NB; ++c) v[s][c] = sprow[NB + c];
+            }
+          }
+        }
+        if (lane == 0) {
+          spiv[j] = pr;
+          if (singular) *sinfo = d;
+        }
+        if (!singular) {
+          const T recip = T(1) / sprow[j];
+#pragma unroll
+          for (int s = 0; s < RPT; ++s) {
+            const int row = kb + lane + 64 * s;
+            if (row > d && row < n) {
+              const T l = v[s][j] * recip;
+              v[s][j] = l;
+#pragma unroll
+              for (int c = j + 1; c < NB; ++c) {
+                const T e = l * sprow[c];
+                v[s][c] = v[s][c] - e;
+              }
+            }
+          }
+        }
+        wave_sync();
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < RPT; ++s) {
+      const int row = kb + lane + 64 * s;
+      if (row < n) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+          if (c < w) {
+            M(row, kb + c) = v[s][c];
+            sL[(row - kb) * NB + c] = v[s][c];
+          }
+      }
+    }
+    wave_sync();
+    if (lane == 0)
+      for (int e = 0; e < w; ++e) ipiv_out[kb + e] = spiv[e];
+    compose_rowmap(kb, w);
+  }
+
+  template <class Mat>
+  __device__ void solve(const Mat &M, int n, int r, int64_t *ipiv_out) {
+    const int nc = n + r;
+    if (t == 0) *sinfo = -1;
+    for (int kb = 0; kb < n; kb += NB) {
+      const int w = n - kb < NB ? n - kb : NB;
+      if (wave == 0) panel(M, n, kb, w, ipiv_out);
+      __syncthreads();
+      // Interchanges on the columns of [A | B] outside the panel.
+      const int cnt = *scnt;
+      for (int ci = t; ci < nc - w; ci += 256) {
+        const int c = ci < kb ? ci : ci + w;
+        T vals[2 * NB];
+#pragma unroll
+        for (int e = 0; e < 2 * NB; ++e)
+          if (e < cnt) vals[e] = M(ssrc[e], c);
+#pragma unroll
+        for (int e = 0; e < 2 * NB; ++e)
+          if (e < cnt) M(srows[e], c) = vals[e];
+      }
+      __syncthreads();
+      // U12 = L11^-1 A12 for the columns right of the panel (and of B).
+      const int c0 = kb + w;
+      for (int c = c0 + t; c < nc; c += 256) {
+        T x[NB];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) x[i] = i < w ? M(kb + i, c) : T(0);
+#pragma unroll
+        for (int i = 1; i < NB; ++i) {
+          T acc = x[i];
+#pragma unroll
+          for (int q = 0; q < i; ++q) acc -= sL[i * NB + q] * x[q];
+          x[i] = acc;
+        }
+#pragma unroll
+        for (int i = 0; i < NB; ++i)
+          if (i < w) {
+            M(kb + i, c) = x[i];
+            sU[i * kBatchLDU + (c - c0)] = x[i];
+          }
+      }
+      __syncthreads();
+      // A22 -= L21 U12 over rows c0..n-1 and columns c0..nc-1.
+      const int rows = n - c0, cols = nc - c0;
+      if (rows > 0) {
+        const int tr = (rows + 3) / 4, tc = (cols + 3) / 4;
+        for (int tile = t; tile < tr * tc; tile += 256) {
+          const int i0 = (tile / tc) * 4, j0 = (tile % tc) * 4;
+          T acc[4][4];
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
+#pragma unroll
+          for (int q = 0; q < NB; ++q) {
+            if (q < w) {
+              T l[4], u[4];
+#pragma unroll
+              for (int a = 0; a < 4; ++a) l[a] = (i0 + a < rows) ? sL[(w + i0 + a) * NB + q] : T(0);
+#pragma unroll
+              for (int b = 0; b < 4; ++b) u[b] = (j0 + b < cols) ? sU[q * kBatchLDU + j0 + b] : T(0);
+#pragma unroll
+              for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b] += l[a] * u[b];
+            }
+          }
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+            if (i0 + a < rows) {
+#pragma unroll
+              for (int b = 0; b < 4; ++b)
+                if (j0 + b < cols) {
+                  T &cv = M(c0 + i0 + a, c0 + j0 + b);
+                  cv = cv - acc[a][b];
+                }
+            }
+        }
+      }
+      __syncthreads();
+    }
+    // Back substitution U X = Y on the right-hand-side columns, blocked from
+    // the bottom: diagonal block (true division), then the rows above.
+    for (int kb = ((n - 1) / NB) * NB; kb >= 0; kb -= NB) {
+      const int w = n - kb < NB ? n - kb : NB;
+      if (t < r) {
+        const int c = n + t;
+        for (int i = w - 1; i >= 0; --i) {
+          const int gi = kb + i;
+          T acc = M(gi, c);
+          for (int q = i + 1; q < w; ++q) acc -= M(gi, kb + q) * M(kb + q, c);
+          M(gi, c) = acc / M(gi, gi);
+        }
+      }
+      __syncthreads();
+      for (int e = t; e < kb * r; e += 256) {
+        const int i = e / r, c = n + e % r;
+        T acc = M(i, c);
+        for (int q = 0; q < w; ++q) acc -= M(i, kb + q) * M(kb + q, c);
+        M(i, c) = acc;
+      }
+      __syncthreads();
+    }
+  }
+};
+
+template <typename T, int NB>
+__global__ void __launch_bounds__(256) batched_gesv_kernel(BatchArgs<T> p) {
+  extern __shared__ unsigned char smem_raw[];
+  T *sL = reinterpret_cast<T *>(smem_raw);
+  T *sU = sL + kBatchMaxN * NB;
+  T *sprow = sU + NB * kBatchLDU;
+  T *sM = sprow + 2 * NB;  // augmented matrix when it fits
+  __shared__ int spiv[NB];
+  __shared__ int srows[2 * NB], ssrc[2 * NB];
+  __shared__ int scnt, sinfo, ssys;
+  BatchSolver<T, NB> S;
+  S.t = threadIdx.x;
+  S.lane = threadIdx.x & 63;
+  S.wave = threadIdx.x >> 6;
+  S.sL = sL;
+  S.sU = sU;
+  S.sprow = sprow;
+  S.spiv = spiv;
+  S.srows = srows;
+  S.ssrc = ssrc;
+  S.scnt = &scnt;
+  S.sinfo = &sinfo;
+  for (;;) {
+    if (threadIdx.x == 0) ssys = atomicAdd(p.queue, 1);
+    __syncthreads();
+    const int idx = ssys;
+    if (idx >= p.count) break;
+    const int sys = p.order[idx];
+    const int n = p.n[sys], r = p.nrhs[sys];
+    T *A = p.a + p.a_off[sys];
+    T *B = p.b + p.b_off[sys];
+    int64_t *piv = p.ipiv + p.p_off[sys];
+    const int nc = n + r;
+    if (n * nc <= p.lds_max_elems) {
+      for (int e = threadIdx.x; e < n * nc; e += 256) {
+        const int i = e / nc, c = e % nc;
+        sM[e] = c < n ? A[i * n + c] : B[i * r + (c - n)];
+      }
+      __syncthreads();
+      S.solve(DenseMat<T>{sM, nc}, n, r, piv);
+      for (int e = threadIdx.x; e < n * nc; e += 256) {
+        const int i = e / nc, c = e % nc;
+        if (c < n)
+          A[i * n + c] = sM[e];
+        else
+          B[i * r + (c - n)] = sM[e];
+      }
+    } else {
+      S.solve(SplitMat<T>{A, B, n, r}, n, r, piv);
+    }
+    if (threadIdx.x == 0) p.info[sys] = sinfo;
+    __syncthreads();
+  }
+}
+
+template <typename T, int NB>
+static size_t batched_lds_bytes(int lds_max_elems) {
+  return sizeof(T) * ((size_t)kBatchMaxN * NB + (size_t)NB * kBatchLDU + 2 * NB + lds_max_elems);
+}
+
+__global__ void zero_int_kernel(int *p) { *p = 0; }
+
+template <typename T>
+hipError_t launch_batched_gesv(lair_context *ctx, const BatchLaunch &bl, T *a, T *b, int64_t *ipiv,
+                               int *info) {
+  BatchArgs<T> p;
+  p.count = bl.count;
+  p.n = bl.n;
+  p.nrhs = bl.nrhs;
+  p.a_off = bl.a_off;
+  p.b_off = bl.b_off;
+  p.p_off = bl.p_off;
+  p.order = bl.order;
+  p.a = a;
+  p.b = b;
+  p.ipiv = ipiv;
+  p.info = info;
+  p.queue = bl.queue;
+  p.lds_max_elems = bl.lds_max_elems;
+  // The queue counter is cleared by a kernel (not a memset), so the pair can
+  // be captured in a graph and replayed.
+  zero_int_kernel<<<1, 1, 0, ctx->stream>>>(bl.queue);
+  const unsigned grid = (unsigned)bl.grid;
+  if (bl.nb == 8) {
+    const size_t shm = batched_lds_bytes<T, 8>(bl.lds_max_elems);
+    batched_gesv_kernel<T, 8><<<grid, 256, shm, ctx->stream>>>(p);
+  } else {
+    const size_t shm = batched_lds_bytes<T, 16>(bl.lds_max_elems);
+    batched_gesv_kernel<T, 16><<<grid, 256, shm, ctx->stream>>>(p);
+  }
+  return hipGetLastError();
+}
+
+// Lifts the dynamic-LDS limit of the batched kernels to the full 160 KB; done
+// before any launch is captured into a graph.
+void batched_prepare() {
+  static bool done = false;
+  if (done) return;
+  for (const void *f : {reinterpret_cast<const void *>(batched_gesv_kernel<double, 8>),
+                        reinterpret_cast<const void *>(batched_gesv_kernel<double, 16>),
+                        reinterpret_cast<const void *>(batched_gesv_kernel<float, 8>),
+                        reinterpret_cast<const void *>(batched_gesv_kernel<float, 16>)})
+    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  (void)hipGetLastError();
+  done = true;
+}
+
+template hipError_t launch_batched_gesv<double>(lair_context *, const BatchLaunch &, double *,
+                                                double *, int64_t *, int *);
+template hipError_t launch_batched_gesv<float>(lair_context *, const BatchLaunch &, float *, float *,
+                                               int64_t *, int *);
+
+size_t batched_lds_bytes_for(int elem_size, int nb, int lds_max_elems) {
+  return (size_t)elem_size *
+         ((size_t)kBatchMaxN * nb + (size_t)nb * kBatchLDU + 2 * nb + lds_max_elems);
+}
+
+}  // namespace lair

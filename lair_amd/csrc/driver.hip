@@ -12,6 +12,8 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <numeric>
+#include <vector>
 
 #include "lair_internal.h"
 
@@ -802,6 +804,274 @@ lair_status lair_dgesv_mixed_device(lair_context *ctx, int64_t n64, int64_t nrhs
   info->precision = LAIR_PREC_FP64;
   info->backward_error = be;
   return LAIR_ERR_NOCONV;
+}
+
+// ---- batched -------------------------------------------------------------
+
+struct lair_batched_plan {
+  lair_context *ctx = nullptr;
+  int prec = 0;
+  int count = 0;
+  void *a = nullptr, *b = nullptr;
+  int64_t *ipiv = nullptr;
+  int *info = nullptr;
+  void *meta = nullptr;  // one device allocation for the metadata arrays
+  lair::BatchLaunch bl;
+  size_t a_elems = 0, b_elems = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static hipError_t batched_launch(lair_batched_plan *pl, const lair::BatchLaunch &bl, void *a, void *b) {
+  if (pl->prec == 0)
+    return launch_batched_gesv<double>(pl->ctx, bl, static_cast<double *>(a), static_cast<double *>(b),
+                                       pl->ipiv, pl->info);
+  return launch_batched_gesv<float>(pl->ctx, bl, static_cast<float *>(a), static_cast<float *>(b),
+                                    pl->ipiv, pl->info);
+}
+
+// Candidate launch shapes: panel width, LDS budget for whole-system-in-LDS,
+// workgroups per CU.  The LDS budget trades systems solved on chip against
+// workgroups resident per CU.
+static std::vector<lair::BatchLaunch> batched_candidates(const lair::BatchLaunch &base, int esize,
+                                                        int ncu) {
+  std::vector<lair::BatchLaunch> out;
+  const size_t lds_cap = 160 * 1024;
+  for (int nb : {16, 8})
+    for (int per_cu : {1, 2})
+      for (int frac : {0, 1, 2}) {
+        lair::BatchLaunch c = base;
+        c.nb = nb;
+        const size_t fixed = batched_lds_bytes_for(esize, nb, 0);
+        const size_t per_wg = lds_cap / per_cu;
+        if (per_wg <= fixed + 1024) continue;
+        const size_t avail = (per_wg - fixed - 1024) / esize;  // elements
+        c.lds_max_elems = frac == 0 ? 0 : (frac == 1 ? (int)(avail / 2) : (int)avail);
+        c.grid = ncu * per_cu;
+        out.push_back(c);
+      }
+  return out;
+}
+
+extern "C" {
+
+lair_status lair_gesv_batched_plan(lair_context *ctx, int32_t precision, int32_t count,
+                                   const int32_t *n, const int32_t *nrhs, void *a, void *b,
+                                   int64_t *ipiv, int32_t *info, int32_t flags,
+                                   lair_batched_plan **plan) {
+  if (!ctx) return -1;
+  if (precision != 0 && precision != 1) return -2;
+  if (count < 0) return -3;
+  if (!plan) return -11;
+  if (count > 0 && (!n || !nrhs)) return -4;
+  for (int i = 0; i < count; ++i) {
+    if (n[i] < 1 || n[i] > 256) return -4;
+    if (nrhs[i] < 1 || nrhs[i] > 16) return -5;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  std::unique_ptr<lair_batched_plan> pl(new lair_batched_plan());
+  pl->ctx = ctx;
+  pl->prec = precision;
+  pl->count = count;
+  pl->a = a;
+  pl->b = b;
+  pl->ipiv = ipiv;
+  pl->info = info;
+  const int esize = precision == 0 ? 8 : 4;
+  std::vector<int64_t> aoff(count), boff(count), poff(count);
+  std::vector<int> order(count);
+  int64_t sa = 0, sb = 0, sp = 0;
+  for (int i = 0; i < count; ++i) {
+    aoff[i] = sa;
+    boff[i] = sb;
+    poff[i] = sp;
+    sa += (int64_t)n[i] * n[i];
+    sb += (int64_t)n[i] * nrhs[i];
+    sp += n[i];
+  }
+  pl->a_elems = (size_t)sa;
+  pl->b_elems = (size_t)sb;
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    return (int64_t)n[x] * n[x] * (n[x] + nrhs[x]) > (int64_t)n[y] * n[y] * (n[y] + nrhs[y]);
+  });
+  const size_t bytes = sizeof(int) * 2 * count + sizeof(int64_t) * 3 * count + sizeof(int) * count + 64;
+  LAIR_HIP(hipMalloc(&pl->meta, bytes));
+  char *m = static_cast<char *>(pl->meta);
+  int64_t *d_aoff = reinterpret_cast<int64_t *>(m);
+  int64_t *d_boff = d_aoff + count;
+  int64_t *d_poff = d_boff + count;
+  int *d_n = reinterpret_cast<int *>(d_poff + count);
+  int *d_r = d_n + count;
+  int *d_order = d_r + count;
+  int *d_queue = d_order + count;
+  if (count > 0) {
+    LAIR_HIP(hipMemcpy(d_aoff, aoff.data(), sizeof(int64_t) * count, hipMemcpyHostToDevice));
+    LAIR_HIP(hipMemcpy(d_boff, boff.data(), sizeof(int64_t) * count, hipMemcpyHostToDevice));
+    LAIR_HIP(hipMemcpy(d_poff, poff.data(), sizeof(int64_t) * count, hipMemcpyHostToDevice));
+    LAIR_HIP(hipMemcpy(d_n, n, sizeof(int) * count, hipMemcpyHostToDevice));
+    LAIR_HIP(hipMemcpy(d_r, nrhs, sizeof(int) * count, hipMemcpyHostToDevice));
+    LAIR_HIP(hipMemcpy(d_order, order.data(), sizeof(int) * count, hipMemcpyHostToDevice));
+  }
+  lair::BatchLaunch bl;
+  bl.count = count;
+  bl.n = d_n;
+  bl.nrhs = d_r;
+  bl.a_off = d_aoff;
+  bl.b_off = d_boff;
+  bl.p_off = d_poff;
+  bl.order = d_order;
+  bl.queue = d_queue;
+  int ncu = 256;
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) ncu = prop.multiProcessorCount;
+  }
+  batched_prepare();
+  std::vector<lair::BatchLaunch> cands = batched_candidates(bl, esize, ncu);
+  lair::BatchLaunch best = cands.front();
+  if ((flags & LAIR_BATCHED_AUTOTUNE) && count > 0) {
+    // Time every candidate on scratch copies (the solve is in place).
+    void *sa_d = nullptr, *sb_d = nullptr;
+    LAIR_HIP(hipMalloc(&sa_d, esize * pl->a_elems));
+    LAIR_HIP(hipMalloc(&sb_d, esize * pl->b_elems));
+    hipEvent_t e0, e1;
+    LAIR_HIP(hipEventCreate(&e0));
+    LAIR_HIP(hipEventCreate(&e1));
+    float best_ms = 1e30f;
+    for (const auto &c : cands) {
+      float cand_ms = 1e30f;
+      for (int rep = 0; rep < 2; ++rep) {
+        LAIR_HIP(hipMemcpyAsync(sa_d, a, esize * pl->a_elems, hipMemcpyDeviceToDevice, ctx->stream));
+        LAIR_HIP(hipMemcpyAsync(sb_d, b, esize * pl->b_elems, hipMemcpyDeviceToDevice, ctx->stream));
+        LAIR_HIP(hipEventRecord(e0, ctx->stream));
+        LAIR_HIP(batched_launch(pl.get(), c, sa_d, sb_d));
+        LAIR_HIP(hipEventRecord(e1, ctx->stream));
+        LAIR_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        LAIR_HIP(hipEventElapsedTime(&ms, e0, e1));
+        cand_ms = std::min(cand_ms, ms);
+      }
+      if (cand_ms < best_ms) {
+        best_ms = cand_ms;
+        best = c;
+      }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(sa_d);
+    hipFree(sb_d);
+  }
+  pl->bl = best;
+  // Capture the (queue reset, persistent kernel) pair.
+  LAIR_HIP(hipStreamSynchronize(ctx->stream));
+  LAIR_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t le = batched_launch(pl.get(), pl->bl, a, b);
+  hipGraph_t g = nullptr;
+  hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+  if (le != hipSuccess) return hip_status(le, "batched capture");
+  if (ce != hipSuccess) return hip_status(ce, "end capture");
+  pl->graph = g;
+  LAIR_HIP(hipGraphInstantiate(&pl->exec, g, nullptr, nullptr, 0));
+  *plan = pl.release();
+  return LAIR_OK;
+}
+
+lair_status lair_gesv_batched_run(lair_batched_plan *plan) {
+  if (!plan) return -1;
+  if (plan->count == 0) return LAIR_OK;
+  std::lock_guard<std::mutex> lk(plan->ctx->mu);
+  LAIR_HIP(hipGraphLaunch(plan->exec, plan->ctx->stream));
+  return LAIR_OK;
+}
+
+lair_status lair_gesv_batched_config(lair_batched_plan *plan, int32_t *nb, int32_t *grid,
+                                     int32_t *lds_elems) {
+  if (!plan) return -1;
+  if (nb) *nb = plan->bl.nb;
+  if (grid) *grid = plan->bl.grid;
+  if (lds_elems) *lds_elems = plan->bl.lds_max_elems;
+  return LAIR_OK;
+}
+
+lair_status lair_gesv_batched_destroy(lair_batched_plan *plan) {
+  if (!plan) return -1;
+  hipStreamSynchronize(plan->ctx->stream);
+  if (plan->exec) hipGraphExecDestroy(plan->exec);
+  if (plan->graph) hipGraphDestroy(plan->graph);
+  if (plan->meta) hipFree(plan->meta);
+  delete plan;
+  return LAIR_OK;
+}
+
+}  // extern "C"
+
+// ---- adjoint -------------------------------------------------------------
+
+static lair_status solve_backward_locked(lair_context *ctx, int n, int nrhs, const double *lu,
+                                         int64_t lda, const int64_t *ipiv, const double *x,
+                                         int64_t ldx, const double *xbar, int64_t ldxb, double *bbar,
+                                         int64_t ldbb, double *abar, int64_t ldab) {
+  hipError_t e = launch_copy_d2d(ctx, n, nrhs, xbar, ldxb, bbar, ldbb);
+  if (e != hipSuccess) return hip_status(e, "copy");
+  lair_status st = getrs_impl<double>(ctx, n, nrhs, lu, lda, ipiv, bbar, ldbb, 1);
+  if (st != LAIR_OK) return st;
+  // X^T (nrhs x n) so that ABAR = -BBAR * X^T is a plain row-major product.
+  LAIR_HIP(ensure(ctx->ws_rhs, sizeof(double) * (size_t)n * nrhs));
+  double *xt = static_cast<double *>(ctx->ws_rhs.ptr);
+  e = launch_transpose_d(ctx, n, nrhs, x, ldx, xt, n);
+  if (e != hipSuccess) return hip_status(e, "transpose");
+  const int slot = prof_begin(ctx, LAIR_KCLASS_GEMM);
+  e = launch_dgemm_tile(ctx, n, n, nrhs, bbar, ldbb, false, xt, n, abar, ldab, true);
+  prof_end(ctx, slot, LAIR_KCLASS_GEMM, 2.0 * n * (double)n * nrhs,
+           sizeof(double) * ((double)n * n + 2.0 * n * nrhs));
+  if (e != hipSuccess) return hip_status(e, "gemm");
+  LAIR_HIP(hipStreamSynchronize(ctx->stream));
+  return LAIR_OK;
+}
+
+lair_status lair_dsolve_backward_device(lair_context *ctx, int64_t n, int64_t nrhs,
+                                        const double *lu, int64_t lda, const int64_t *ipiv,
+                                        const double *x, int64_t ldx, const double *xbar,
+                                        int64_t ldxb, double *bbar, int64_t ldbb, double *abar,
+                                        int64_t ldab) {
+  if (!ctx) return -1;
+  if (n < 0) return -2;
+  if (nrhs < 0) return -3;
+  if (n == 0 || nrhs == 0) return LAIR_OK;
+  if (n > INT32_MAX || nrhs > INT32_MAX) return LAIR_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return solve_backward_locked(ctx, (int)n, (int)nrhs, lu, lda, ipiv, x, ldx, xbar, ldxb, bbar,
+                               ldbb, abar, ldab);
+}
+
+lair_status lair_dsolve_backward(int64_t n, int64_t nrhs, const double *lu, const int64_t *ipiv,
+                                 const double *x, const double *xbar, double *bbar, double *abar) {
+  if (n < 0) return -1;
+  if (nrhs < 0) return -2;
+  if (n == 0 || nrhs == 0) return LAIR_OK;
+  if (!lu || !ipiv || !x || !xbar || !bbar || !abar) return -3;
+  lair_context *ctx = default_ctx();
+  if (!ctx) return LAIR_ERR_DEVICE;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  const size_t nn = (size_t)n * n, nr = (size_t)n * nrhs;
+  // One device allocation: lu, abar (n x n each), x, xbar, bbar (n x nrhs), ipiv.
+  const size_t bytes = sizeof(double) * (2 * nn + 3 * nr) + sizeof(int64_t) * n;
+  LAIR_HIP(ensure(ctx->ws_tmp, bytes));
+  double *d_lu = static_cast<double *>(ctx->ws_tmp.ptr);
+  double *d_ab = d_lu + nn, *d_x = d_ab + nn, *d_xb = d_x + nr, *d_bb = d_xb + nr;
+  int64_t *d_piv = reinterpret_cast<int64_t *>(d_bb + nr);
+  LAIR_HIP(hipMemcpyAsync(d_lu, lu, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream));
+  LAIR_HIP(hipMemcpyAsync(d_x, x, sizeof(double) * nr, hipMemcpyHostToDevice, ctx->stream));
+  LAIR_HIP(hipMemcpyAsync(d_xb, xbar, sizeof(double) * nr, hipMemcpyHostToDevice, ctx->stream));
+  LAIR_HIP(hipMemcpyAsync(d_piv, ipiv, sizeof(int64_t) * n, hipMemcpyHostToDevice, ctx->stream));
+  lair_status st = solve_backward_locked(ctx, (int)n, (int)nrhs, d_lu, n, d_piv, d_x, nrhs, d_xb,
+                                         nrhs, d_bb, nrhs, d_ab, n);
+  if (st != LAIR_OK) return st;
+  LAIR_HIP(hipMemcpyAsync(bbar, d_bb, sizeof(double) * nr, hipMemcpyDeviceToHost, ctx->stream));
+  LAIR_HIP(hipMemcpyAsync(abar, d_ab, sizeof(double) * nn, hipMemcpyDeviceToHost, ctx->stream));
+  LAIR_HIP(hipStreamSynchronize(ctx->stream));
+  return LAIR_OK;
 }
 
 // ---- instrumentation -----------------------------------------------------

@@ -40,17 +40,43 @@ __device__ __forceinline__ void tile_coords(int nbm, int nbn, int &tm, int &tn) 
   tn = lin % nbn;
 }
 
-// ------------------------------------------------------------------ FP64 --
+// ------------------------------------------------------- FP64 and FP32 --
 
-template <bool TRANSA>
+template <typename T>
+struct Mfma16;
+template <>
+struct Mfma16<double> {
+  typedef d4 acc_t;
+  typedef d2 pair_t;
+  static __device__ __forceinline__ d4 op(double a, double b, d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // row of accumulator element r of lane l within the 16 x 16 block
+  static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <>
+struct Mfma16<float> {
+  typedef f4 acc_t;
+  typedef float pair_t __attribute__((ext_vector_type(2)));
+  static __device__ __forceinline__ f4 op(float a, float b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+// STORE = false: C -= op(A) B (trailing update); STORE = true: C = -op(A) B
+// (C is only written, e.g. the adjoint's A-gradient).
+template <typename T, bool TRANSA, bool STORE>
 __global__ void __launch_bounds__(256)
-    dgemm_tile_kernel(int M, int N, int K, const double *__restrict__ A, int64_t lda,
-                      const double *__restrict__ B, int64_t ldb, double *__restrict__ C, int64_t ldc) {
+    gemm_tile_kernel(int M, int N, int K, const T *__restrict__ A, int64_t lda,
+                     const T *__restrict__ B, int64_t ldb, T *__restrict__ C, int64_t ldc) {
+  typedef typename Mfma16<T>::acc_t acc_t;
+  typedef typename Mfma16<T>::pair_t d2;
   constexpr int BK = 16;
   constexpr int PA = kTileM + 4;  // padded row of the k-major A stage
   constexpr int PB = kTileN + 4;
-  __shared__ double As[2][BK][PA];  // As[k][i]
-  __shared__ double Bs[2][BK][PB];  // Bs[k][j]
+  __shared__ T As[2][BK][PA];  // As[k][i]
+  __shared__ T Bs[2][BK][PB];  // Bs[k][j]
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int nbm = (M + kTileM - 1) / kTileM, nbn = (N + kTileN - 1) / kTileN;
   int tm, tn;
@@ -61,7 +87,7 @@ __global__ void __launch_bounds__(256)
   // Global fetch mapping: A (not transposed): thread -> row t/2, k (t%2)*8..+8;
   // A transposed (stored K x M): thread -> k t/16, rows (t%16)*8..+8;
   // B: thread -> k t/16, cols (t%16)*8..+8.
-  double ra[8], rb[8];
+  T ra[8], rb[8];
   auto fetch = [&](int k0) {
     if (!TRANSA) {
       const int i = t >> 1, kk = (t & 1) * 8;
@@ -74,8 +100,8 @@ __global__ void __launch_bounds__(256)
           ra[q] = v[0];
           ra[q + 1] = v[1];
         } else {
-          ra[q] = (gi < M && gk < K) ? A[(int64_t)gi * lda + gk] : 0.0;
-          ra[q + 1] = (gi < M && gk + 1 < K) ? A[(int64_t)gi * lda + gk + 1] : 0.0;
+          ra[q] = (gi < M && gk < K) ? A[(int64_t)gi * lda + gk] : T(0);
+          ra[q + 1] = (gi < M && gk + 1 < K) ? A[(int64_t)gi * lda + gk + 1] : T(0);
         }
       }
     } else {
@@ -84,7 +110,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const int gi = bm + i0 + q;
-        ra[q] = (gk < K && gi < M) ? A[(int64_t)gk * lda + gi] : 0.0;
+        ra[q] = (gk < K && gi < M) ? A[(int64_t)gk * lda + gi] : T(0);
       }
     }
     const int kb = t >> 4, j0 = (t & 15) * 8;
@@ -97,8 +123,8 @@ __global__ void __launch_bounds__(256)
         rb[q] = v[0];
         rb[q + 1] = v[1];
       } else {
-        rb[q] = (gk < K && gj < N) ? B[(int64_t)gk * ldb + gj] : 0.0;
-        rb[q + 1] = (gk < K && gj + 1 < N) ? B[(int64_t)gk * ldb + gj + 1] : 0.0;
+        rb[q] = (gk < K && gj < N) ? B[(int64_t)gk * ldb + gj] : T(0);
+        rb[q + 1] = (gk < K && gj + 1 < N) ? B[(int64_t)gk * ldb + gj + 1] : T(0);
       }
     }
   };
@@ -117,11 +143,11 @@ __global__ void __launch_bounds__(256)
     for (int q = 0; q < 8; q += 2) *reinterpret_cast<d2 *>(&Bs[buf][kb][j0 + q]) = (d2){rb[q], rb[q + 1]};
   };
 
-  d4 acc[4][4];
+  acc_t acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < 4; ++j) acc[i][j] = (acc_t){T(0), T(0), T(0), T(0)};
 
   const int nstage = (K + BK - 1) / BK;
   fetch(0);
@@ -132,7 +158,7 @@ __global__ void __launch_bounds__(256)
     if (s + 1 < nstage) fetch((s + 1) * BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
-      double av[4], bv[4];
+      T av[4], bv[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) av[i] = As[buf][kk + (lane >> 4)][wm + i * 16 + (lane & 15)];
 #pragma unroll
@@ -141,7 +167,7 @@ __global__ void __launch_bounds__(256)
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = Mfma16<T>::op(av[i], bv[j], acc[i][j]);
     }
     if (s + 1 < nstage) stash(buf ^ 1);
     __syncthreads();
@@ -153,23 +179,216 @@ __global__ void __launch_bounds__(256)
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gi = bm + wm + i * 16 + (lane >> 4) + 4 * r;
+        const int gi = bm + wm + i * 16 + Mfma16<T>::row(lane, r);
         const int gj = bn + wn + j * 16 + (lane & 15);
         if (gi < M && gj < N) {
-          double *cp = C + (int64_t)gi * ldc + gj;
-          *cp = *cp - acc[i][j][r];
+          T *cp = C + (int64_t)gi * ldc + gj;
+          if (STORE)
+            *cp = -acc[i][j][r];
+          else
+            *cp = *cp - acc[i][j][r];
         }
       }
 }
 
 hipError_t launch_dgemm_tile(lair_context *ctx, int M, int N, int K, const double *A, int64_t lda,
-                             bool transA, const double *B, int64_t ldb, double *C, int64_t ldc) {
+                             bool transA, const double *B, int64_t ldb, double *C, int64_t ldc,
+                             bool store) {
+  const int nbm = (M + kTileM - 1) / kTileM, nbn = (N + kTileN - 1) / kTileN;
+  const unsigned grid = (unsigned)(nbm * nbn);
+  hipStream_t st = ctx->stream;
+  if (store) {
+    if (transA)
+      gemm_tile_kernel<double, true, true><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
+    else
+      gemm_tile_kernel<double, false, true><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
+  } else {
+    if (transA)
+      gemm_tile_kernel<double, true, false><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
+    else
+      gemm_tile_kernel<double, false, false><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_sgemm_tile(lair_context *ctx, int M, int N, int K, const float *A, int64_t lda,
+                             bool transA, const float *B, int64_t ldb, float *C, int64_t ldc) {
   const int nbm = (M + kTileM - 1) / kTileM, nbn = (N + kTileN - 1) / kTileN;
   const unsigned grid = (unsigned)(nbm * nbn);
   if (transA)
-    dgemm_tile_kernel<true><<<grid, 256, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C, ldc);
+    gemm_tile_kernel<float, true, false><<<grid, 256, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C, ldc);
   else
-    dgemm_tile_kernel<false><<<grid, 256, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C, ldc);
+    gemm_tile_kernel<float, false, false><<<grid, 256, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C, ldc);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------- BF16 splits ------
+
+// Splits x into NS bf16 terms whose sum carries 8*NS significand bits; each
+// remainder x - (t0 + ... + ts) is exact in FP32.
+template <int NS>
+__device__ __forceinline__ void split3(float x, __bf16 *out) {
+  float r = x;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const __bf16 h = (__bf16)r;
+    out[s] = h;
+    r = r - (float)h;
+  }
+}
+
+// C -= A B with FP32 operands split into NS bf16 terms.  NS = 3 keeps the six
+// products whose magnitude is at least 2^-16 of the leading one (hh, hm, mh,
+// mm, hl, lh), issued smallest first into the FP32 MFMA accumulators.
+// A is M x K row-major; B is K x N row-major.  Stage: BK = 32 (one
+// 16x16x32 MFMA step).  LDS holds the split planes [s][i][k] of A and
+// [s][j][k] of B (k contiguous: each lane reads 8 consecutive k); the next
+// stage's raw FP32 values are prefetched into registers during the MFMAs.
+template <int NS>
+__global__ void __launch_bounds__(256)
+    bgemm_tile_kernel(int M, int N, int K, const float *__restrict__ A, int64_t lda,
+                      const float *__restrict__ B, int64_t ldb, float *__restrict__ C, int64_t ldc) {
+  constexpr int BK = 32;
+  constexpr int PK = BK + 8;  // padded k extent (bf16 elements)
+  __shared__ __bf16 As[NS][kTileM][PK];
+  __shared__ __bf16 Bs[NS][kTileN][PK];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int nbm = (M + kTileM - 1) / kTileM, nbn = (N + kTileN - 1) / kTileN;
+  int tm, tn;
+  tile_coords(nbm, nbn, tm, tn);
+  const int bm = tm * kTileM, bn = tn * kTileN;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  // A: thread -> row t/2, k (t%2)*16 .. +16.  B: thread -> j 4*(t%32) .. +4,
+  // k rows 2*(t/32) + {0, 1} and + 16.
+  float ra[16], rb[16];
+  auto fetch = [&](int k0) {
+    const int i = t >> 1, kk = (t & 1) * 16;
+    const int gi = bm + i;
+#pragma unroll
+    for (int q = 0; q < 16; q += 4) {
+      const int gk = k0 + kk + q;
+      if (gi < M && gk + 3 < K) {
+        const f4 v = *reinterpret_cast<const f4 *>(A + (int64_t)gi * lda + gk);
+        ra[q] = v[0];
+        ra[q + 1] = v[1];
+        ra[q + 2] = v[2];
+        ra[q + 3] = v[3];
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) ra[q + e] = (gi < M && gk + e < K) ? A[(int64_t)gi * lda + gk + e] : 0.f;
+      }
+    }
+    const int j0 = (t & 31) * 4, kp = (t >> 5) * 2;
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int gk = k0 + kp + e + 16 * h;
+        const int gj = bn + j0;
+        float *dst = rb + (h * 2 + e) * 4;
+        if (gk < K && gj + 3 < N) {
+          const f4 v = *reinterpret_cast<const f4 *>(B + (int64_t)gk * ldb + gj);
+          dst[0] = v[0];
+          dst[1] = v[1];
+          dst[2] = v[2];
+          dst[3] = v[3];
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) dst[q] = (gk < K && gj + q < N) ? B[(int64_t)gk * ldb + gj + q] : 0.f;
+        }
+      }
+  };
+  auto stash = [&]() {
+    const int i = t >> 1, kk = (t & 1) * 16;
+#pragma unroll
+    for (int q = 0; q < 16; q += 8) {
+      bf8 part[NS];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        __bf16 sp[NS];
+        split3<NS>(ra[q + e], sp);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) part[s][e] = sp[s];
+      }
+#pragma unroll
+      for (int s = 0; s < NS; ++s) *reinterpret_cast<bf8 *>(&As[s][i][kk + q]) = part[s];
+    }
+    const int j0 = (t & 31) * 4, kp = (t >> 5) * 2;
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        __bf16 s0[NS], s1[NS];
+        split3<NS>(rb[(h * 2 + 0) * 4 + q], s0);
+        split3<NS>(rb[(h * 2 + 1) * 4 + q], s1);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+          typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+          *reinterpret_cast<bf2 *>(&Bs[s][j0 + q][kp + 16 * h]) = (bf2){s0[s], s1[s]};
+        }
+      }
+  };
+
+  f4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
+
+  const int nstage = (K + BK - 1) / BK;
+  fetch(0);
+  for (int s = 0; s < nstage; ++s) {
+    __syncthreads();
+    stash();
+    __syncthreads();
+    if (s + 1 < nstage) fetch((s + 1) * BK);
+    bf8 av[NS][4], bv[NS][4];
+#pragma unroll
+    for (int p = 0; p < NS; ++p) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        av[p][i] = *reinterpret_cast<const bf8 *>(&As[p][wm + i * 16 + (lane & 15)][(lane >> 4) * 8]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        bv[p][j] = *reinterpret_cast<const bf8 *>(&Bs[p][wn + j * 16 + (lane & 15)][(lane >> 4) * 8]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (NS == 3) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[2][i], bv[0][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[0][i], bv[2][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[1][i], bv[1][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[1][i], bv[0][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[0][i], bv[1][j], acc[i][j], 0, 0, 0);
+        }
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[0][i], bv[0][j], acc[i][j], 0, 0, 0);
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = bm + wm + i * 16 + 4 * (lane >> 4) + r;
+        const int gj = bn + wn + j * 16 + (lane & 15);
+        if (gi < M && gj < N) {
+          float *cp = C + (int64_t)gi * ldc + gj;
+          *cp = *cp - acc[i][j][r];
+        }
+      }
+}
+
+hipError_t launch_bgemm_tile(lair_context *ctx, int ns, int M, int N, int K, const float *A,
+                             int64_t lda, const float *B, int64_t ldb, float *C, int64_t ldc) {
+  const int nbm = (M + kTileM - 1) / kTileM, nbn = (N + kTileN - 1) / kTileN;
+  const unsigned grid = (unsigned)(nbm * nbn);
+  if (ns == 3)
+    bgemm_tile_kernel<3><<<grid, 256, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C, ldc);
+  else
+    bgemm_tile_kernel<1><<<grid, 256, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C, ldc);
   return hipGetLastError();
 }
 

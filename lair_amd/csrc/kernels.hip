@@ -836,6 +836,17 @@ hipError_t launch_gemm_minus<float>(lair_context *ctx, int mode, int M, int N, i
                                     const float *A, int64_t lda, bool transA, const float *B,
                                     int64_t ldb, float *C, int64_t ldc, int kclass) {
   if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
+  if (M >= 256 && N >= 128 && K >= 32 && !(transA && mode != kGemmF32)) {
+    const int slot = prof_begin(ctx, kclass);
+    hipError_t e;
+    if (mode == kGemmF32)
+      e = launch_sgemm_tile(ctx, M, N, K, A, lda, transA, B, ldb, C, ldc);
+    else
+      e = launch_bgemm_tile(ctx, mode == kGemmBF16x6 ? 3 : 1, M, N, K, A, lda, B, ldb, C, ldc);
+    prof_end(ctx, slot, kclass, 2.0 * M * N * K,
+             (double)sizeof(float) * (2.0 * M * N + (double)M * K + (double)K * N));
+    return e;
+  }
   dim3 grid((N + 63) / 64, (M + 63) / 64);
   const int slot = prof_begin(ctx, kclass);
   hipStream_t st = ctx->stream;
@@ -955,6 +966,31 @@ hipError_t launch_copy_d2d(lair_context *ctx, int rows, int cols, const double *
   if (total <= 0) return hipSuccess;
   copy2d_kernel<double, double><<<(unsigned)((total + 255) / 256), 256, 0, ctx->stream>>>(
       rows, cols, x, ldx, y, ldy);
+  return hipGetLastError();
+}
+
+// y (cols x rows) = x^T (x is rows x cols), 32 x 32 tiles through LDS.
+__global__ void transpose_d_kernel(int rows, int cols, const double *x, int64_t ldx, double *y,
+                                   int64_t ldy) {
+  __shared__ double tile[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 32 x 8
+  for (int r = ty; r < 32; r += 8) {
+    const int gr = by + r, gc = bx + tx;
+    tile[r][tx] = (gr < rows && gc < cols) ? x[(int64_t)gr * ldx + gc] : 0.0;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int gc = bx + r, gr = by + tx;  // y[gc][gr] = x[gr][gc]
+    if (gc < cols && gr < rows) y[(int64_t)gc * ldy + gr] = tile[tx][r];
+  }
+}
+
+hipError_t launch_transpose_d(lair_context *ctx, int rows, int cols, const double *x, int64_t ldx,
+                              double *y, int64_t ldy) {
+  if (rows <= 0 || cols <= 0) return hipSuccess;
+  dim3 grid((cols + 31) / 32, (rows + 31) / 32);
+  transpose_d_kernel<<<grid, 256, 0, ctx->stream>>>(rows, cols, x, ldx, y, ldy);
   return hipGetLastError();
 }
 

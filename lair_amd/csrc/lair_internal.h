@@ -149,7 +149,33 @@ hipError_t launch_trsv_dataflow(lair_context *ctx, const T *lu, int64_t lda, con
 
 // 128x128-tile MFMA trailing updates (gemm_kernels.hip).
 hipError_t launch_dgemm_tile(lair_context *ctx, int M, int N, int K, const double *A, int64_t lda,
-                             bool transA, const double *B, int64_t ldb, double *C, int64_t ldc);
+                             bool transA, const double *B, int64_t ldb, double *C, int64_t ldc,
+                             bool store = false);
+hipError_t launch_sgemm_tile(lair_context *ctx, int M, int N, int K, const float *A, int64_t lda,
+                             bool transA, const float *B, int64_t ldb, float *C, int64_t ldc);
+// ns = 3: six-product bf16 split (FP32-level); ns = 1: single bf16 product.
+hipError_t launch_bgemm_tile(lair_context *ctx, int ns, int M, int N, int K, const float *A,
+                             int64_t lda, const float *B, int64_t ldb, float *C, int64_t ldc);
+
+// Persistent batched solver (batched.hip): device metadata and launch shape.
+struct BatchLaunch {
+  int count = 0;
+  const int *n = nullptr;
+  const int *nrhs = nullptr;
+  const int64_t *a_off = nullptr;
+  const int64_t *b_off = nullptr;
+  const int64_t *p_off = nullptr;
+  const int *order = nullptr;
+  int *queue = nullptr;
+  int lds_max_elems = 0;  // largest augmented matrix solved in LDS
+  int nb = 16;            // panel width (8 or 16)
+  int grid = 256;         // persistent workgroups
+};
+template <typename T>
+hipError_t launch_batched_gesv(lair_context *ctx, const BatchLaunch &bl, T *a, T *b, int64_t *ipiv,
+                               int *info);
+size_t batched_lds_bytes_for(int elem_size, int nb, int lds_max_elems);
+void batched_prepare();
 
 hipError_t launch_fill_i64(lair_context *ctx, int64_t *p, int64_t n, int64_t v);
 hipError_t launch_fill_u32(lair_context *ctx, uint32_t *p, int64_t n, uint32_t v);
@@ -168,6 +194,9 @@ hipError_t launch_copy_s2d(lair_context *ctx, int rows, int cols, const float *x
                            double *y, int64_t ldy);
 hipError_t launch_copy_d2d(lair_context *ctx, int rows, int cols, const double *x, int64_t ldx,
                            double *y, int64_t ldy);
+// y (cols x rows) = x^T.
+hipError_t launch_transpose_d(lair_context *ctx, int rows, int cols, const double *x, int64_t ldx,
+                              double *y, int64_t ldy);
 // x += (double) d
 hipError_t launch_axpy_s2d(lair_context *ctx, int rows, int cols, const float *d, int64_t ldd,
                            double *x, int64_t ldx);
