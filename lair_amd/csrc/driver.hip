@@ -124,7 +124,7 @@ static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a,
   if (m64 < 0) return -2;
   if (n64 < 0) return -3;
   if (lda < std::max<int64_t>(n64, 1)) return -5;
-  if (m64 > (int64_t)kMaxPanelWG * 1024 || n64 > INT32_MAX) {
+  if (m64 > (int64_t)kMaxPanelWG * 2048 || n64 > INT32_MAX) {
     set_error("matrix too large for the cooperative panel");
     return LAIR_ERR_UNSUPPORTED;
   }

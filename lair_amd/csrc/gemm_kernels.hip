@@ -67,16 +67,16 @@ struct Mfma16<float> {
 // STORE = false: C -= op(A) B (trailing update); STORE = true: C = -op(A) B
 // (C is only written, e.g. the adjoint's A-gradient).
 template <typename T, bool TRANSA, bool STORE>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 2)
     gemm_tile_kernel(int M, int N, int K, const T *__restrict__ A, int64_t lda,
                      const T *__restrict__ B, int64_t ldb, T *__restrict__ C, int64_t ldc) {
   typedef typename Mfma16<T>::acc_t acc_t;
   typedef typename Mfma16<T>::pair_t d2;
   constexpr int BK = 16;
-  constexpr int PA = kTileM + 4;  // padded row of the k-major A stage
+  constexpr int PA = BK + 1;      // padded row of the A stage (16 lanes read 16 rows)
   constexpr int PB = kTileN + 4;
-  __shared__ T As[2][BK][PA];  // As[k][i]
-  __shared__ T Bs[2][BK][PB];  // Bs[k][j]
+  __shared__ T As[2][kTileM][PA];  // As[i][k]
+  __shared__ T Bs[2][BK][PB];      // Bs[k][j]
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int nbm = (M + kTileM - 1) / kTileM, nbn = (N + kTileN - 1) / kTileN;
   int tm, tn;
@@ -132,11 +132,11 @@ __global__ void __launch_bounds__(256)
     if (!TRANSA) {
       const int i = t >> 1, kk = (t & 1) * 8;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) As[buf][kk + q][i] = ra[q];
+      for (int q = 0; q < 8; ++q) As[buf][i][kk + q] = ra[q];
     } else {
       const int kk = t >> 4, i0 = (t & 15) * 8;
 #pragma unroll
-      for (int q = 0; q < 8; q += 2) *reinterpret_cast<d2 *>(&As[buf][kk][i0 + q]) = (d2){ra[q], ra[q + 1]};
+      for (int q = 0; q < 8; ++q) As[buf][i0 + q][kk] = ra[q];
     }
     const int kb = t >> 4, j0 = (t & 15) * 8;
 #pragma unroll
@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(256)
     for (int kk = 0; kk < BK; kk += 4) {
       T av[4], bv[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) av[i] = As[buf][kk + (lane >> 4)][wm + i * 16 + (lane & 15)];
+      for (int i = 0; i < 4; ++i) av[i] = As[buf][wm + i * 16 + (lane & 15)][kk + (lane >> 4)];
 #pragma unroll
       for (int j = 0; j < 4; ++j) bv[j] = Bs[buf][kk + (lane >> 4)][wn + j * 16 + (lane & 15)];
 #pragma unroll
@@ -173,22 +173,29 @@ __global__ void __launch_bounds__(256)
     __syncthreads();
   }
 
+  // Epilogue: per 16-row block, all 16 loads of C are issued before any store.
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 4; ++i) {
+    T cv[4][4];
+    if (!STORE) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gi = bm + wm + i * 16 + Mfma16<T>::row(lane, r);
+          const int gj = bn + wn + j * 16 + (lane & 15);
+          cv[j][r] = (gi < M && gj < N) ? C[(int64_t)gi * ldc + gj] : T(0);
+        }
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int gi = bm + wm + i * 16 + Mfma16<T>::row(lane, r);
         const int gj = bn + wn + j * 16 + (lane & 15);
-        if (gi < M && gj < N) {
-          T *cp = C + (int64_t)gi * ldc + gj;
-          if (STORE)
-            *cp = -acc[i][j][r];
-          else
-            *cp = *cp - acc[i][j][r];
-        }
+        if (gi < M && gj < N) C[(int64_t)gi * ldc + gj] = STORE ? -acc[i][j][r] : cv[j][r] - acc[i][j][r];
       }
+  }
 }
 
 hipError_t launch_dgemm_tile(lair_context *ctx, int M, int N, int K, const double *A, int64_t lda,
@@ -245,7 +252,7 @@ __device__ __forceinline__ void split3(float x, __bf16 *out) {
 // [s][j][k] of B (k contiguous: each lane reads 8 consecutive k); the next
 // stage's raw FP32 values are prefetched into registers during the MFMAs.
 template <int NS>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 2)
     bgemm_tile_kernel(int M, int N, int K, const float *__restrict__ A, int64_t lda,
                       const float *__restrict__ B, int64_t ldb, float *__restrict__ C, int64_t ldc) {
   constexpr int BK = 32;
@@ -367,18 +374,25 @@ __global__ void __launch_bounds__(256)
       }
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 4; ++i) {
+    float cv[4][4];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int gi = bm + wm + i * 16 + 4 * (lane >> 4) + r;
         const int gj = bn + wn + j * 16 + (lane & 15);
-        if (gi < M && gj < N) {
-          float *cp = C + (int64_t)gi * ldc + gj;
-          *cp = *cp - acc[i][j][r];
-        }
+        cv[j][r] = (gi < M && gj < N) ? C[(int64_t)gi * ldc + gj] : 0.f;
       }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = bm + wm + i * 16 + 4 * (lane >> 4) + r;
+        const int gj = bn + wn + j * 16 + (lane & 15);
+        if (gi < M && gj < N) C[(int64_t)gi * ldc + gj] = cv[j][r] - acc[i][j][r];
+      }
+  }
 }
 
 hipError_t launch_bgemm_tile(lair_context *ctx, int ns, int M, int N, int K, const float *A,

@@ -421,24 +421,34 @@ hipError_t launch_panel(lair_context *ctx, T *a, int64_t lda, int row0, int m, i
   pa.drec = ctx->drec;
   pa.seq = ctx->seq;
   pa.err = ctx->err;
-  constexpr int NT = 256;
-  int rpt;
-  if (rows <= NT)
+  // Rows per workgroup: 256 threads x {1, 2, 4} rows, or 512 threads x 4 rows
+  // for tall panels (fewer workgroups take part in each column's exchange).
+  int nt, rpt;
+  if (rows <= 256) {
+    nt = 256;
     rpt = 1;
-  else if (rows <= 2 * NT)
+  } else if (rows <= 512) {
+    nt = 256;
     rpt = 2;
-  else
+  } else if (rows <= 1024) {
+    nt = 256;
     rpt = 4;
-  const int R = NT * rpt;
+  } else {
+    nt = 512;
+    rpt = 4;
+  }
+  const int R = nt * rpt;
   pa.P = (rows + R - 1) / R;
   if (pa.P > kMaxPanelWG) return hipErrorInvalidValue;
   const int slot = prof_begin(ctx, LAIR_KCLASS_PANEL);
-  if (rpt == 1)
-    panel_coop_kernel<T, kLeafW, 1, NT><<<pa.P, NT, 0, ctx->stream>>>(pa);
+  if (nt == 512)
+    panel_coop_kernel<T, kLeafW, 4, 512><<<pa.P, 512, 0, ctx->stream>>>(pa);
+  else if (rpt == 1)
+    panel_coop_kernel<T, kLeafW, 1, 256><<<pa.P, 256, 0, ctx->stream>>>(pa);
   else if (rpt == 2)
-    panel_coop_kernel<T, kLeafW, 2, NT><<<pa.P, NT, 0, ctx->stream>>>(pa);
+    panel_coop_kernel<T, kLeafW, 2, 256><<<pa.P, 256, 0, ctx->stream>>>(pa);
   else
-    panel_coop_kernel<T, kLeafW, 4, NT><<<pa.P, NT, 0, ctx->stream>>>(pa);
+    panel_coop_kernel<T, kLeafW, 4, 256><<<pa.P, 256, 0, ctx->stream>>>(pa);
   const double fl = (double)pa.npiv * (double)rows * (double)wcols;
   prof_end(ctx, slot, LAIR_KCLASS_PANEL, fl, 2.0 * sizeof(T) * rows * wcols);
   ctx->seq += (uint32_t)kLeafW;
