@@ -149,41 +149,72 @@ static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a,
     if (e != hipSuccess) return hip_status(e, "fill");
     ctx->seq = 1;
   }
-  for (int k = 0; k < kmax; k += (int)nb) {
-    const int kb = std::min<int>((int)nb, kmax - k);
+  // Factors the panel of columns [k, k + kb) on the context's stream.
+  auto factor_panel = [&](int k, int kb) -> hipError_t {
     for (int c = k; c < k + kb; c += kLeafW) {
       const int w = std::min(kLeafW, k + kb - c);
-      e = launch_panel<T>(ctx, a, lda, c, m, c, w, ipiv);
-      if (e != hipSuccess) return hip_status(e, "panel");
-      if (kb > w) {
-        e = launch_rowswap<T>(ctx, a, lda, k, k + kb, c, c + w, ipiv, c, c + w, nullptr);
-        if (e != hipSuccess) return hip_status(e, "rowswap");
-      }
+      // The leaf kernel also applies its interchanges to the other columns
+      // of the panel and solves the leaf's U rows right of it.
+      hipError_t e2 = launch_panel<T>(ctx, a, lda, c, m, c, w, ipiv, k + kb, k);
+      if (e2 != hipSuccess) return e2;
       const int rest = k + kb - (c + w);
-      if (rest > 0) {
-        e = launch_trsm_llu<T>(ctx, w, rest, a + (int64_t)c * lda + c, lda,
-                               a + (int64_t)c * lda + c + w, lda);
-        if (e != hipSuccess) return hip_status(e, "trsm");
-        e = launch_gemm_minus<T>(ctx, mode, m - (c + w), rest, w, a + (int64_t)(c + w) * lda + c,
-                                 lda, false, a + (int64_t)c * lda + c + w, lda,
-                                 a + (int64_t)(c + w) * lda + c + w, lda, LAIR_KCLASS_GEMM_PANEL);
-        if (e != hipSuccess) return hip_status(e, "gemm");
+      if (rest > 0 && m > c + w) {
+        e2 = launch_gemm_minus<T>(ctx, mode, m - (c + w), rest, w, a + (int64_t)(c + w) * lda + c, lda,
+                                  false, a + (int64_t)c * lda + c + w, lda,
+                                  a + (int64_t)(c + w) * lda + c + w, lda, LAIR_KCLASS_GEMM_PANEL);
+        if (e2 != hipSuccess) return e2;
       }
     }
-    e = launch_rowswap<T>(ctx, a, lda, 0, n, k, k + kb, ipiv, k, k + kb, perm);
-    if (e != hipSuccess) return hip_status(e, "rowswap");
-    const int right = n - (k + kb);
-    if (right > 0) {
-      e = launch_trsm_llu<T>(ctx, kb, right, a + (int64_t)k * lda + k, lda,
-                             a + (int64_t)k * lda + k + kb, lda);
-      if (e != hipSuccess) return hip_status(e, "trsm");
-      if (m - (k + kb) > 0) {
-        e = launch_gemm_minus<T>(ctx, mode, m - (k + kb), right, kb, a + (int64_t)(k + kb) * lda + k,
-                                 lda, false, a + (int64_t)k * lda + k + kb, lda,
-                                 a + (int64_t)(k + kb) * lda + k + kb, lda);
-        if (e != hipSuccess) return hip_status(e, "gemm");
-      }
+    return hipSuccess;
+  };
+  // Brings columns [c0, c1) up to date with panel [k, k + kb): interchanges,
+  // U12 = L11^-1 A12, A22 -= L21 U12.
+  auto update_cols = [&](int k, int kb, int c0, int c1) -> hipError_t {
+    if (c1 <= c0) return hipSuccess;
+    hipError_t e2 = launch_rowswap<T>(ctx, a, lda, c0, c1, c0, c0, ipiv, k, k + kb, nullptr);
+    if (e2 != hipSuccess) return e2;
+    e2 = launch_trsm_llu<T>(ctx, kb, c1 - c0, a + (int64_t)k * lda + k, lda, a + (int64_t)k * lda + c0,
+                            lda);
+    if (e2 != hipSuccess) return e2;
+    if (m > k + kb)
+      e2 = launch_gemm_minus<T>(ctx, mode, m - (k + kb), c1 - c0, kb, a + (int64_t)(k + kb) * lda + k,
+                                lda, false, a + (int64_t)k * lda + c0, lda,
+                                a + (int64_t)(k + kb) * lda + c0, lda);
+    return e2;
+  };
+  // Look-ahead of depth one.  Stream 1 (the context's stream) carries the
+  // critical path: panel k+1's columns are updated first, then panel k+1 is
+  // factored.  Stream 2 meanwhile applies panel k to everything else: the
+  // columns left of it (interchanges only, and perm) and the columns right of
+  // panel k+1 (interchanges, trsm, gemm).  Stream 1 waits for stream 2's
+  // update of panel k before it touches panel k+2's columns.
+  if (kmax > 0) {
+    e = factor_panel(0, std::min<int>((int)nb, kmax));
+    if (e != hipSuccess) return hip_status(e, "panel");
+  }
+  for (int k = 0; k < kmax; k += (int)nb) {
+    const int kb = std::min<int>((int)nb, kmax - k);
+    const bool has_next = k + kb < kmax;
+    const int nxt1 = has_next ? std::min<int>(k + kb + (int)nb, n) : k + kb;
+    LAIR_HIP(hipEventRecord(ctx->ev_panel, ctx->stream));
+    {
+      SideStream side(ctx);
+      LAIR_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_panel, 0));
+      // Left columns: interchanges only (and the row permutation).
+      e = launch_rowswap<T>(ctx, a, lda, 0, k, 0, 0, ipiv, k, k + kb, perm);
+      if (e != hipSuccess) return hip_status(e, "rowswap");
+      e = update_cols(k, kb, nxt1, n);
+      if (e != hipSuccess) return hip_status(e, "update");
+      LAIR_HIP(hipEventRecord(ctx->ev_bulk, ctx->stream));
     }
+    if (has_next) {
+      e = update_cols(k, kb, k + kb, nxt1);
+      if (e != hipSuccess) return hip_status(e, "update");
+      e = factor_panel(k + kb, std::min<int>((int)nb, kmax - (k + kb)));
+      if (e != hipSuccess) return hip_status(e, "panel");
+    }
+    // Panel k+1's update of the columns right of it reads what stream 2 wrote.
+    LAIR_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_bulk, 0));
   }
   if (singular) {
     int32_t err = 0;
@@ -442,6 +473,12 @@ lair_status lair_context_create(lair_context **out) {
   LAIR_HIP(hipMalloc(&ctx->swap_rows, sizeof(int) * kMaxSwapRows));
   LAIR_HIP(hipMalloc(&ctx->swap_src, sizeof(int) * kMaxSwapRows));
   LAIR_HIP(hipMalloc(&ctx->swap_count, sizeof(int)));
+  LAIR_HIP(hipMalloc(&ctx->swap_rows_alt, sizeof(int) * kMaxSwapRows));
+  LAIR_HIP(hipMalloc(&ctx->swap_src_alt, sizeof(int) * kMaxSwapRows));
+  LAIR_HIP(hipMalloc(&ctx->swap_count_alt, sizeof(int)));
+  LAIR_HIP(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+  LAIR_HIP(hipEventCreateWithFlags(&ctx->ev_panel, hipEventDisableTiming));
+  LAIR_HIP(hipEventCreateWithFlags(&ctx->ev_bulk, hipEventDisableTiming));
   LAIR_HIP(hipMalloc(&ctx->norms, sizeof(double) * 4096));
   LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->rec), 4 * kMaxPanelWG * 64, 0u));
   LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->drec), 4 * 64, 0u));
@@ -464,6 +501,15 @@ lair_status lair_context_destroy(lair_context *ctx) {
   hipFree(ctx->swap_rows);
   hipFree(ctx->swap_src);
   hipFree(ctx->swap_count);
+  hipFree(ctx->swap_rows_alt);
+  hipFree(ctx->swap_src_alt);
+  hipFree(ctx->swap_count_alt);
+  if (ctx->stream2) {
+    hipStreamSynchronize(ctx->stream2);
+    hipStreamDestroy(ctx->stream2);
+  }
+  if (ctx->ev_panel) hipEventDestroy(ctx->ev_panel);
+  if (ctx->ev_bulk) hipEventDestroy(ctx->ev_bulk);
   for (Buffer *b : {&ctx->ws_a32, &ctx->ws_lu64, &ctx->ws_ipiv, &ctx->ws_x32, &ctx->ws_r,
                     &ctx->ws_tmp, &ctx->ws_rhs, &ctx->ws_perm, &ctx->ws_linv, &ctx->ws_uinv,
                     &ctx->ws_flags, &ctx->ws_tmp2})

@@ -63,6 +63,10 @@ struct PanelArgs {
   unsigned long long *drec;  // [2][kRecWords] tagged words of the diagonal row
   uint32_t seq;
   int32_t *err;
+  int stride;  // launched slots per workgroup; only slot 0 of each group works
+  int rest1;   // panel columns [col0 + wcols, rest1) receive the leaf's
+               // interchanges and their U rows (L11^-1 A12) in this launch;
+  int left0;   // columns [left0, col0) receive the interchanges only
 };
 
 __device__ __forceinline__ unsigned long long tagged(uint32_t tag, uint32_t data) {
@@ -163,6 +167,7 @@ struct PanelLeaf {
   T (*s_prow)[W];             // [2][W] pivot row
   T (*s_drow)[W];             // [2][W] diagonal row before the interchange
   int *s_win;                 // [2]
+  int *s_piv;                 // [W] the leaf's pivot rows (workgroup 0)
   uint32_t *s_rec;            // [P][L] record words, then [W*WPE] diagonal words
   int *s_bail;
   int g, t, lane, wave, base;
@@ -204,7 +209,7 @@ struct PanelLeaf {
           s_rec[total + lane] = (uint32_t)x;
         }
         if (__all(ok)) break;
-        if (++spins > kSpinLimit || ld_agent(pa.err) != 0) {
+        if ((++spins & 255) == 0 && (spins > kSpinLimit || ld_agent(pa.err) != 0)) {
           bail = true;
           break;
         }
@@ -315,6 +320,7 @@ struct PanelLeaf {
       }
     }
     if (g == 0 && t == 0) {
+      s_piv[J] = pr;
       pa.ipiv[d] = pr;
       if (singular) *pa.sing = d;
     }
@@ -363,7 +369,8 @@ __global__ void __launch_bounds__(NT) panel_coop_kernel(PanelArgs<T> pa) {
   constexpr int NWAVE = NT / 64;
   constexpr int R = NT * RPT;
   constexpr int L = PanelLeaf<T, W, RPT, NT>::L;
-  const int g = blockIdx.x, t = threadIdx.x;
+  if (blockIdx.x % pa.stride != 0) return;  // idle slot (placement only)
+  const int g = blockIdx.x / pa.stride, t = threadIdx.x;
   const int base = pa.row0 + g * R;
 
   T v[RPT][W];
@@ -381,10 +388,11 @@ __global__ void __launch_bounds__(NT) panel_coop_kernel(PanelArgs<T> pa) {
   __shared__ T s_prow[2][W];
   __shared__ T s_drow[2][W];
   __shared__ int s_win[2];
+  __shared__ int s_piv[W];
   __shared__ uint32_t s_rec[kMaxPanelWG * L + W * (sizeof(T) / 4)];
   __shared__ int s_bail;
 
-  PanelLeaf<T, W, RPT, NT> leaf{pa, v, s_wv, s_wr, s_wrow, s_prow, s_drow, s_win, s_rec, &s_bail,
+  PanelLeaf<T, W, RPT, NT> leaf{pa, v, s_wv, s_wr, s_wrow, s_prow, s_drow, s_win, s_piv, s_rec, &s_bail,
                                 g, t, t & 63, t >> 6, base};
   LAIR_PANEL_MARK(0, 0);
   if (!ColumnLoop<0, W>::run(leaf)) return;
@@ -400,11 +408,82 @@ __global__ void __launch_bounds__(NT) panel_coop_kernel(PanelArgs<T> pa) {
         if (w < pa.wcols) dst[w] = v[s][w];
     }
   }
+  if (g != 0 || (pa.rest1 <= pa.col0 + pa.wcols && pa.left0 >= pa.col0)) return;
+
+  // Rest of the panel (workgroup 0 only; no other workgroup touches these
+  // columns during the launch).  The leaf's npiv interchanges compose into a
+  // row map (rows[e] receives the old contents of src[e]); the first npiv
+  // entries are the leaf's diagonal rows, which become U12 = L11^-1 A12
+  // (getrf.rs:270-278: laswp, then trsm with the unit-lower L11).
+  __shared__ int s_mrow[2 * W], s_msrc[2 * W], s_mn;
+  __shared__ T s_l11[W][W];
+  __shared__ T s_buf[2 * W][64];
+  const int npiv = pa.npiv;
+  if (t < npiv) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) s_l11[t][w] = v[0][w];  // row col0 + t lives in thread t
+  }
+  if (t == 0) {
+    int nx = npiv;
+    for (int e = 0; e < npiv; ++e) {
+      s_mrow[e] = pa.col0 + e;
+      s_msrc[e] = pa.col0 + e;
+    }
+    for (int e = 0; e < npiv; ++e) {
+      const int p = s_piv[e];
+      const int k = pa.col0 + e;
+      if (p == k) continue;
+      int ep = -1;
+      if (p < pa.col0 + npiv) {
+        ep = p - pa.col0;
+      } else {
+        for (int q = npiv; q < nx; ++q)
+          if (s_mrow[q] == p) ep = q;
+        if (ep < 0) {
+          ep = nx++;
+          s_mrow[ep] = p;
+          s_msrc[ep] = p;
+        }
+      }
+      const int tmp = s_msrc[e];
+      s_msrc[e] = s_msrc[ep];
+      s_msrc[ep] = tmp;
+    }
+    s_mn = nx;
+  }
+  __syncthreads();
+  const int nent = s_mn;
+  const int c0 = pa.col0 + pa.wcols;
+  const int ql = t & 63, eg = t >> 6;
+  // Columns [left0, col0) (interchanges only), then [c0, rest1).
+  const int nleft = pa.left0 < pa.col0 ? pa.col0 - pa.left0 : 0;
+  const int ntot = nleft + (pa.rest1 > c0 ? pa.rest1 - c0 : 0);
+  for (int cb = 0; cb < ntot; cb += 64) {
+    const int qi = cb + ql;
+    const bool act = qi < ntot;
+    const int q = qi < nleft ? pa.left0 + qi : c0 + (qi - nleft);
+    const bool solve = qi >= nleft;
+    for (int e = eg; e < nent; e += NWAVE)
+      if (act) s_buf[e][ql] = pa.a[(int64_t)s_msrc[e] * pa.lda + q];
+    __syncthreads();
+    if (eg == 0 && act && solve) {
+      for (int i = 1; i < npiv; ++i) {
+        T acc = s_buf[i][ql];
+        for (int p = 0; p < i; ++p) acc -= s_l11[i][p] * s_buf[p][ql];
+        s_buf[i][ql] = acc;
+      }
+    }
+    __syncthreads();
+    for (int e = eg; e < nent; e += NWAVE)
+      if (act && ((solve && e < npiv) || s_msrc[e] != s_mrow[e]))
+        pa.a[(int64_t)s_mrow[e] * pa.lda + q] = s_buf[e][ql];
+    __syncthreads();
+  }
 }
 
 template <typename T>
 hipError_t launch_panel(lair_context *ctx, T *a, int64_t lda, int row0, int m, int col0, int wcols,
-                        int64_t *ipiv) {
+                        int64_t *ipiv, int rest1, int left0) {
   const int rows = m - row0;
   if (rows <= 0 || wcols <= 0) return hipSuccess;
   PanelArgs<T> pa;
@@ -421,6 +500,10 @@ hipError_t launch_panel(lair_context *ctx, T *a, int64_t lda, int row0, int m, i
   pa.drec = ctx->drec;
   pa.seq = ctx->seq;
   pa.err = ctx->err;
+  pa.stride = ctx->panel_xcd_stride;
+  pa.rest1 = rest1;
+  pa.left0 = left0;
+  if ((rest1 > col0 + wcols || left0 < col0) && row0 != col0) return hipErrorInvalidValue;
   // Rows per workgroup: 256 threads x {1, 2, 4} rows, or 512 threads x 4 rows
   // for tall panels (fewer workgroups take part in each column's exchange).
   int nt, rpt;
@@ -441,14 +524,15 @@ hipError_t launch_panel(lair_context *ctx, T *a, int64_t lda, int row0, int m, i
   pa.P = (rows + R - 1) / R;
   if (pa.P > kMaxPanelWG) return hipErrorInvalidValue;
   const int slot = prof_begin(ctx, LAIR_KCLASS_PANEL);
+  const unsigned grid = (unsigned)(pa.P * pa.stride);
   if (nt == 512)
-    panel_coop_kernel<T, kLeafW, 4, 512><<<pa.P, 512, 0, ctx->stream>>>(pa);
+    panel_coop_kernel<T, kLeafW, 4, 512><<<grid, 512, 0, ctx->stream>>>(pa);
   else if (rpt == 1)
-    panel_coop_kernel<T, kLeafW, 1, 256><<<pa.P, 256, 0, ctx->stream>>>(pa);
+    panel_coop_kernel<T, kLeafW, 1, 256><<<grid, 256, 0, ctx->stream>>>(pa);
   else if (rpt == 2)
-    panel_coop_kernel<T, kLeafW, 2, 256><<<pa.P, 256, 0, ctx->stream>>>(pa);
+    panel_coop_kernel<T, kLeafW, 2, 256><<<grid, 256, 0, ctx->stream>>>(pa);
   else
-    panel_coop_kernel<T, kLeafW, 4, 256><<<pa.P, 256, 0, ctx->stream>>>(pa);
+    panel_coop_kernel<T, kLeafW, 4, 256><<<grid, 256, 0, ctx->stream>>>(pa);
   const double fl = (double)pa.npiv * (double)rows * (double)wcols;
   prof_end(ctx, slot, LAIR_KCLASS_PANEL, fl, 2.0 * sizeof(T) * rows * wcols);
   ctx->seq += (uint32_t)kLeafW;
@@ -456,9 +540,9 @@ hipError_t launch_panel(lair_context *ctx, T *a, int64_t lda, int row0, int m, i
 }
 
 template hipError_t launch_panel<double>(lair_context *, double *, int64_t, int, int, int, int,
-                                         int64_t *);
+                                         int64_t *, int, int);
 template hipError_t launch_panel<float>(lair_context *, float *, int64_t, int, int, int, int,
-                                        int64_t *);
+                                        int64_t *, int, int);
 
 // ---------------------------------------------------------------- laswp ---
 

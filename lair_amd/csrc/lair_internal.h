@@ -72,12 +72,20 @@ struct lair_context {
   unsigned long long *rec = nullptr;
   unsigned long long *drec = nullptr;
   uint32_t seq = 0;            // tag base of the next panel launch
+  int panel_xcd_stride = 1;    // launched slots per panel workgroup (see launch_panel)
   int32_t *err = nullptr;      // set when a bounded wait expires
   int64_t *sing = nullptr;     // last zero-pivot column (device)
   double *norms = nullptr;     // scratch for norm reductions (uint64-ordered doubles)
 
-  // Row-interchange map of the current panel (device).
+  // Row-interchange map of the current panel (device); the *_alt set belongs
+  // to the look-ahead stream (swapped in by lair::SideStream).
   int *swap_rows = nullptr, *swap_src = nullptr, *swap_count = nullptr;
+  int *swap_rows_alt = nullptr, *swap_src_alt = nullptr, *swap_count_alt = nullptr;
+
+  // Look-ahead: the trailing update of panel k runs on stream2 while the
+  // critical path (next panel's columns, next panel) continues on stream.
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_panel = nullptr, ev_bulk = nullptr;
 
   // Grow-only workspaces.
   lair::Buffer ws_a32, ws_lu64, ws_ipiv, ws_x32, ws_r, ws_tmp, ws_rhs;
@@ -97,6 +105,20 @@ struct lair_context {
 
 namespace lair {
 
+// Redirects the context's launches to stream2 (and its row-map buffers) for
+// the lifetime of the object.
+struct SideStream {
+  lair_context *ctx;
+  explicit SideStream(lair_context *c) : ctx(c) { swap(); }
+  ~SideStream() { swap(); }
+  void swap() {
+    std::swap(ctx->stream, ctx->stream2);
+    std::swap(ctx->swap_rows, ctx->swap_rows_alt);
+    std::swap(ctx->swap_src, ctx->swap_src_alt);
+    std::swap(ctx->swap_count, ctx->swap_count_alt);
+  }
+};
+
 // Records (when profiling) a start event for a kernel of class kclass.
 int prof_begin(lair_context *ctx, int kclass);
 void prof_end(lair_context *ctx, int slot, int kclass, double flops, double bytes = 0.0);
@@ -107,7 +129,7 @@ hipError_t ensure(Buffer &b, size_t bytes);
 
 template <typename T>
 hipError_t launch_panel(lair_context *ctx, T *a, int64_t lda, int row0, int m, int col0, int wcols,
-                        int64_t *ipiv);
+                        int64_t *ipiv, int rest1, int left0);
 
 // Row interchanges ipiv[k0..k1) (forward, or reverse when rev) applied to the
 // columns [c0, c1) minus [s0, s1) of a (row-major, lda).
