@@ -30,7 +30,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "n=16384 solve time: BF16/FP32-MFMA LU+IR vs FP64 LU at equal backward error"
 PEAK = {"hbm_gbs": 8000.0, "f64": 78.6, "f32": 157.3, "bf16": 2500.0}
-PREC_NAMES = {"fp32": 1, "bf16x6": 2}
+# Mixed-precision variants: factorization arithmetic, and "+tp" for tournament
+# pivoting of the FP32 panel (LAIR_PIVOT_TOURNAMENT) instead of partial pivoting.
+PREC_NAMES = {"fp32": 1, "bf16x6": 2, "fp32+tp": 1 | 0x100, "bf16x6+tp": 2 | 0x100}
 EPS = float(np.finfo(np.float64).eps)
 
 
@@ -85,7 +87,7 @@ def main():
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--nrhs", type=int, default=1)
     ap.add_argument("--kappa", type=float, default=1e6)
-    ap.add_argument("--precisions", default="bf16x6,fp32")
+    ap.add_argument("--precisions", default="bf16x6+tp,fp32+tp,bf16x6,fp32")
     ap.add_argument("--cpu-sample-n", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -188,7 +190,7 @@ def main():
     # Roofline of the trailing update: the lower of the MFMA and HBM ceilings.
     g_ms, g_cnt, g_fl, g_by = hr["gemm"]
     mfma_peak = PEAK["bf16"] if head.startswith("bf16") else PEAK["f32"]
-    products = 6 if head == "bf16x6" else 1
+    products = 6 if head.startswith("bf16x6") else 1
     t_mfma = g_fl * products / (mfma_peak * 1e12)
     t_hbm = g_by / (PEAK["hbm_gbs"] * 1e9)
     if t_hbm >= t_mfma:
@@ -221,7 +223,7 @@ def main():
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "bf16x6-mfma+f64-ir" if head == "bf16x6" else "f32-mfma+f64-ir",
+        "dtype": ("bf16x6-mfma+f64-ir" if head.startswith("bf16x6") else "f32-mfma+f64-ir"),
         "data": "synthetic",
         "config": {
             "workload": f"dgesv n={n} nrhs={nrhs} kappa={args.kappa:g} (Haar U,V; log-spaced s)",

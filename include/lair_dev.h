@@ -48,6 +48,12 @@ typedef struct lair_context lair_context;
                               (3-way bf16 split of each operand), FP32 accumulation */
 #define LAIR_PREC_BF16 3   /* FP32 storage, single BF16 MFMA product (diagnostic; does not
                               converge on ill-conditioned systems) */
+/* Flag OR'd into the precision of the mixed solver: choose each 16-column
+ * leaf's pivot rows by tournament pivoting (a reduction tree of small
+ * partial-pivoting eliminations, one cross-workgroup exchange per leaf)
+ * instead of one global argmax per column.  The pivot order then differs from
+ * partial pivoting; the refined FP64 answer meets the same tolerance. */
+#define LAIR_PIVOT_TOURNAMENT 0x100
 
 /* ------------------------------------------------------------ context --- */
 
@@ -114,6 +120,15 @@ lair_status lair_dgetrf_device(lair_context *ctx, int64_t m, int64_t n, double *
                                int64_t *ipiv, int64_t nb, int64_t *singular);
 lair_status lair_sgetrf_device(lair_context *ctx, int64_t m, int64_t n, float *a, int64_t lda,
                                int64_t *ipiv, int64_t nb, int64_t *singular);
+/* FP32 LU on device data with a choice of pivoting and trailing-update
+ * arithmetic, as used inside the mixed solver: `pivoting` is 0 (partial
+ * pivoting, as lair_sgetrf_device) or LAIR_PIVOT_TOURNAMENT; `prec` is
+ * LAIR_PREC_FP32, LAIR_PREC_BF16X6 or LAIR_PREC_BF16.  ipiv follows the
+ * LAPACK convention in both cases, so P A = L U with P from ipiv.  No
+ * reference counterpart (the reference has partial pivoting only). */
+lair_status lair_sgetrf_pivot_device(lair_context *ctx, int64_t m, int64_t n, float *a, int64_t lda,
+                                     int64_t *ipiv, int64_t nb, int32_t pivoting, int32_t prec,
+                                     int64_t *singular);
 
 /* getrs on device data: b is n x nrhs row-major (leading dimension ldb),
  * overwritten with the solution of A X = B (trans = 0) or A^T X = B. */
@@ -136,7 +151,8 @@ typedef struct lair_ir_info {
 
 /* Mixed-precision solve A X = B from device-resident FP64 data (row-major,
  * lda, ldb, ldx): the factorization runs in `prec` (LAIR_PREC_FP32 /
- * LAIR_PREC_BF16X6 / LAIR_PREC_BF16) and FP64 accuracy is recovered by
+ * LAIR_PREC_BF16X6 / LAIR_PREC_BF16, optionally | LAIR_PIVOT_TOURNAMENT;
+ * LAIR_PREC_FP64 runs the FP64 solver) and FP64 accuracy is recovered by
  * iterative refinement with the residual, correction update and norms in FP64.
  * Stops when the normwise backward error is <= tol (tol <= 0 selects
  * n * DBL_EPSILON) or after max_iter steps; a system that does not converge is

@@ -29,6 +29,7 @@ PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_BF16X6 = 2
 PREC_BF16 = 3
+PIVOT_TOURNAMENT = 0x100
 
 KCLASS = {"panel": 0, "laswp": 1, "trsm": 2, "gemm": 3, "getrs": 4, "residual": 5, "convert": 6,
           "gemm_panel": 7}
@@ -65,6 +66,7 @@ SIGNATURES = [
     ("lair_dsolve", i32, [i64, i64, vp, i64, i64, i64, vp, vp]),
     ("lair_dgetrf_device", i32, [vp, i64, i64, vp, i64, vp, i64, P64]),
     ("lair_sgetrf_device", i32, [vp, i64, i64, vp, i64, vp, i64, P64]),
+    ("lair_sgetrf_pivot_device", i32, [vp, i64, i64, vp, i64, vp, i64, i32, i32, P64]),
     ("lair_dgetrs_device", i32, [vp, i64, i64, vp, i64, vp, vp, i64, i32]),
     ("lair_sgetrs_device", i32, [vp, i64, i64, vp, i64, vp, vp, i64, i32]),
     ("lair_dgesv_mixed_device", i32,

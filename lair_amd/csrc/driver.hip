@@ -120,7 +120,7 @@ static lair_status hip_status(hipError_t e, const char *what) {
 template <typename T>
 static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a, int64_t lda,
                               int64_t *ipiv, int64_t nb, int mode, int64_t *singular,
-                              int *perm = nullptr) {
+                              int *perm = nullptr, bool tournament = false) {
   if (m64 < 0) return -2;
   if (n64 < 0) return -3;
   if (lda < std::max<int64_t>(n64, 1)) return -5;
@@ -147,6 +147,8 @@ static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a,
     if (e != hipSuccess) return hip_status(e, "fill");
     e = launch_fill_u32(ctx, reinterpret_cast<uint32_t *>(ctx->drec), 4 * 64, 0u);
     if (e != hipSuccess) return hip_status(e, "fill");
+    e = launch_fill_u32(ctx, reinterpret_cast<uint32_t *>(ctx->crec), 2 * kCaluWords, 0u);
+    if (e != hipSuccess) return hip_status(e, "fill");
     ctx->seq = 1;
   }
   // Factors the panel of columns [k, k + kb) on the context's stream.
@@ -155,7 +157,15 @@ static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a,
       const int w = std::min(kLeafW, k + kb - c);
       // The leaf kernel also applies its interchanges to the other columns
       // of the panel and solves the leaf's U rows right of it.
-      hipError_t e2 = launch_panel<T>(ctx, a, lda, c, m, c, w, ipiv, k + kb, k);
+      hipError_t e2;
+      if constexpr (std::is_same<T, float>::value) {
+        if (tournament && m - c <= kCaluMaxWG * 2048)
+          e2 = launch_panel_calu(ctx, a, lda, c, m, w, ipiv, k + kb, k);
+        else
+          e2 = launch_panel<T>(ctx, a, lda, c, m, c, w, ipiv, k + kb, k);
+      } else {
+        e2 = launch_panel<T>(ctx, a, lda, c, m, c, w, ipiv, k + kb, k);
+      }
       if (e2 != hipSuccess) return e2;
       const int rest = k + kb - (c + w);
       if (rest > 0 && m > c + w) {
@@ -468,6 +478,7 @@ lair_status lair_context_create(lair_context **out) {
   LAIR_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   LAIR_HIP(hipMalloc(&ctx->rec, sizeof(unsigned long long) * 2 * kMaxPanelWG * 64));
   LAIR_HIP(hipMalloc(&ctx->drec, sizeof(unsigned long long) * 2 * 64));
+  LAIR_HIP(hipMalloc(&ctx->crec, sizeof(unsigned long long) * kCaluWords));
   LAIR_HIP(hipMalloc(&ctx->err, sizeof(int32_t)));
   LAIR_HIP(hipMalloc(&ctx->sing, sizeof(int64_t)));
   LAIR_HIP(hipMalloc(&ctx->swap_rows, sizeof(int) * kMaxSwapRows));
@@ -476,12 +487,21 @@ lair_status lair_context_create(lair_context **out) {
   LAIR_HIP(hipMalloc(&ctx->swap_rows_alt, sizeof(int) * kMaxSwapRows));
   LAIR_HIP(hipMalloc(&ctx->swap_src_alt, sizeof(int) * kMaxSwapRows));
   LAIR_HIP(hipMalloc(&ctx->swap_count_alt, sizeof(int)));
+  // The look-ahead stream carries the bulk trailing updates; their grids are
+  // capped (see launch_gemm_minus's max_wg) so that kReservedCUs CUs stay free
+  // for the critical path on `stream`.
   LAIR_HIP(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+  {
+    hipDeviceProp_t prop;
+    LAIR_HIP(hipGetDeviceProperties(&prop, ctx->device));
+    ctx->num_cu = prop.multiProcessorCount;
+  }
   LAIR_HIP(hipEventCreateWithFlags(&ctx->ev_panel, hipEventDisableTiming));
   LAIR_HIP(hipEventCreateWithFlags(&ctx->ev_bulk, hipEventDisableTiming));
   LAIR_HIP(hipMalloc(&ctx->norms, sizeof(double) * 4096));
   LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->rec), 4 * kMaxPanelWG * 64, 0u));
   LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->drec), 4 * 64, 0u));
+  LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->crec), 2 * kCaluWords, 0u));
   ctx->seq = 1;
   LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->err), 1, 0u));
   LAIR_HIP(hipStreamSynchronize(ctx->stream));
@@ -495,6 +515,7 @@ lair_status lair_context_destroy(lair_context *ctx) {
   for (auto e : ctx->event_pool) hipEventDestroy(e);
   hipFree(ctx->rec);
   hipFree(ctx->drec);
+  hipFree(ctx->crec);
   hipFree(ctx->err);
   hipFree(ctx->sing);
   hipFree(ctx->norms);
@@ -566,6 +587,18 @@ lair_status lair_sgetrf_device(lair_context *ctx, int64_t m, int64_t n, float *a
   if (!ctx) return -1;
   std::lock_guard<std::mutex> lk(ctx->mu);
   return getrf_impl<float>(ctx, m, n, a, lda, ipiv, nb, kGemmF32, singular);
+}
+
+lair_status lair_sgetrf_pivot_device(lair_context *ctx, int64_t m, int64_t n, float *a, int64_t lda,
+                                     int64_t *ipiv, int64_t nb, int32_t pivoting, int32_t prec,
+                                     int64_t *singular) {
+  if (!ctx) return -1;
+  if (pivoting != 0 && pivoting != LAIR_PIVOT_TOURNAMENT) return -8;
+  if (prec != LAIR_PREC_FP32 && prec != LAIR_PREC_BF16X6 && prec != LAIR_PREC_BF16) return -9;
+  const int mode = prec == LAIR_PREC_FP32 ? kGemmF32 : (prec == LAIR_PREC_BF16X6 ? kGemmBF16x6 : kGemmBF16);
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return getrf_impl<float>(ctx, m, n, a, lda, ipiv, nb, mode, singular, nullptr,
+                           pivoting == LAIR_PIVOT_TOURNAMENT);
 }
 
 lair_status lair_dgetrs_device(lair_context *ctx, int64_t n, int64_t nrhs, const double *lu,
@@ -745,6 +778,8 @@ lair_status lair_dgesv_mixed_device(lair_context *ctx, int64_t n64, int64_t nrhs
   if (!ctx) return -1;
   if (n64 < 0) return -2;
   if (nrhs64 < 0 || nrhs64 > 2000) return -3;
+  const bool tournament = (prec & LAIR_PIVOT_TOURNAMENT) != 0;
+  prec &= ~LAIR_PIVOT_TOURNAMENT;
   if (prec < LAIR_PREC_FP64 || prec > LAIR_PREC_BF16) return -11;
   if (max_iter < 0 || max_iter > 30) return -12;
   lair_ir_info local;
@@ -787,7 +822,7 @@ lair_status lair_dgesv_mixed_device(lair_context *ctx, int64_t n64, int64_t nrhs
   if (e != hipSuccess) return hip_status(e, "fill");
   e = launch_convert_d2s_norm(ctx, n, n, a, lda, a32, ld32, norms);
   if (e != hipSuccess) return hip_status(e, "convert");
-  lair_status st = getrf_impl<float>(ctx, n, n, a32, ld32, piv, 0, mode, nullptr, perm);
+  lair_status st = getrf_impl<float>(ctx, n, n, a32, ld32, piv, 0, mode, nullptr, perm, tournament);
   if (st != LAIR_OK) return st;
   SolveAux<float> aux;
   if (n > kExactOrderMax) {

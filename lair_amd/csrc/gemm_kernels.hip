@@ -13,6 +13,8 @@
 // Operand lane layouts (16 x 16 x K MFMAs): lane l supplies A[l%16][k-group
 // l/16] and B[k-group l/16][l%16]; FP64 16x16x4 returns C[l/16 + 4r][l%16],
 // FP32 16x16x4 and BF16 16x16x32 return C[4(l/16) + r][l%16].
+#include <algorithm>
+
 #include "lair_internal.h"
 
 namespace lair {
@@ -28,8 +30,7 @@ constexpr int kTileN = 128;
 // Workgroup -> tile mapping.  Consecutive workgroups are dispatched to
 // different XCDs (round-robin over 8); grouping tiles so that the 8 tiles
 // sharing an A row-panel run on one XCD lets that XCD's L2 serve the panel.
-__device__ __forceinline__ void tile_coords(int nbm, int nbn, int &tm, int &tn) {
-  const int id = blockIdx.x;
+__device__ __forceinline__ void tile_coords(int id, int nbm, int nbn, int &tm, int &tn) {
   const int total = nbm * nbn;
   int lin = id;
   if (total % 8 == 0) {
@@ -66,8 +67,12 @@ struct Mfma16<float> {
 
 // STORE = false: C -= op(A) B (trailing update); STORE = true: C = -op(A) B
 // (C is only written, e.g. the adjoint's A-gradient).
-template <typename T, bool TRANSA, bool STORE>
-__global__ void __launch_bounds__(256, 2)
+//
+// TPW > 1 (capped grids): a workgroup of 256 * TPW threads computes TPW
+// horizontally adjacent tiles at once (one group of four waves each, own LDS
+// stages), so one workgroup per CU has the occupancy of TPW workgroups.
+template <typename T, bool TRANSA, bool STORE, int TPW>
+__global__ void __launch_bounds__(256 * TPW, TPW == 1 ? 2 : 1)
     gemm_tile_kernel(int M, int N, int K, const T *__restrict__ A, int64_t lda,
                      const T *__restrict__ B, int64_t ldb, T *__restrict__ C, int64_t ldc) {
   typedef typename Mfma16<T>::acc_t acc_t;
@@ -75,12 +80,22 @@ __global__ void __launch_bounds__(256, 2)
   constexpr int BK = 16;
   constexpr int PA = BK + 1;      // padded row of the A stage (16 lanes read 16 rows)
   constexpr int PB = kTileN + 4;
-  __shared__ T As[2][kTileM][PA];  // As[i][k]
-  __shared__ T Bs[2][BK][PB];      // Bs[k][j]
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  __shared__ T As_all[TPW][2][kTileM][PA];  // As[i][k]
+  __shared__ T Bs_all[TPW][2][BK][PB];      // Bs[k][j]
+  const int h = threadIdx.x >> 8;
+  auto &As = As_all[h];
+  auto &Bs = Bs_all[h];
+  const int t = threadIdx.x & 255, lane = t & 63, wave = t >> 6;
   const int nbm = (M + kTileM - 1) / kTileM, nbn = (N + kTileN - 1) / kTileN;
-  int tm, tn;
-  tile_coords(nbm, nbn, tm, tn);
+  const int nbnp = (nbn + TPW - 1) / TPW;
+  const int ntile = nbm * nbnp;
+  // A capped grid loops over the tiles (same XCD grouping: gridDim.x is a
+  // multiple of 8 whenever it is below the tile count).  A tile past the
+  // right edge (tn >= nbn) has every load and store masked.
+  for (int id = blockIdx.x; id < ntile; id += gridDim.x) {
+  int tm, tnp;
+  tile_coords(id, nbm, nbnp, tm, tnp);
+  const int tn = tnp * TPW + h;
   const int bm = tm * kTileM, bn = tn * kTileN;
   const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
 
@@ -196,24 +211,31 @@ __global__ void __launch_bounds__(256, 2)
         if (gi < M && gj < N) C[(int64_t)gi * ldc + gj] = STORE ? -acc[i][j][r] : cv[j][r] - acc[i][j][r];
       }
   }
+  }
 }
 
 hipError_t launch_dgemm_tile(lair_context *ctx, int M, int N, int K, const double *A, int64_t lda,
                              bool transA, const double *B, int64_t ldb, double *C, int64_t ldc,
                              bool store) {
   const int nbm = (M + kTileM - 1) / kTileM, nbn = (N + kTileN - 1) / kTileN;
-  const unsigned grid = (unsigned)(nbm * nbn);
   hipStream_t st = ctx->stream;
+  if (ctx->bulk_wg_cap > 0 && !transA && !store) {
+    const int ntile = nbm * ((nbn + 1) / 2);
+    const unsigned grid = (unsigned)std::min(ntile, ctx->bulk_wg_cap);
+    gemm_tile_kernel<double, false, false, 2><<<grid, 512, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
+    return hipGetLastError();
+  }
+  const unsigned grid = (unsigned)(nbm * nbn);
   if (store) {
     if (transA)
-      gemm_tile_kernel<double, true, true><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
+      gemm_tile_kernel<double, true, true, 1><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
     else
-      gemm_tile_kernel<double, false, true><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
+      gemm_tile_kernel<double, false, true, 1><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
   } else {
     if (transA)
-      gemm_tile_kernel<double, true, false><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
+      gemm_tile_kernel<double, true, false, 1><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
     else
-      gemm_tile_kernel<double, false, false><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
+      gemm_tile_kernel<double, false, false, 1><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
   }
   return hipGetLastError();
 }
@@ -221,11 +243,18 @@ hipError_t launch_dgemm_tile(lair_context *ctx, int M, int N, int K, const doubl
 hipError_t launch_sgemm_tile(lair_context *ctx, int M, int N, int K, const float *A, int64_t lda,
                              bool transA, const float *B, int64_t ldb, float *C, int64_t ldc) {
   const int nbm = (M + kTileM - 1) / kTileM, nbn = (N + kTileN - 1) / kTileN;
+  if (ctx->bulk_wg_cap > 0 && !transA) {
+    const int ntile = nbm * ((nbn + 3) / 4);
+    const unsigned grid = (unsigned)std::min(ntile, ctx->bulk_wg_cap);
+    gemm_tile_kernel<float, false, false, 4><<<grid, 1024, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C,
+                                                                              ldc);
+    return hipGetLastError();
+  }
   const unsigned grid = (unsigned)(nbm * nbn);
   if (transA)
-    gemm_tile_kernel<float, true, false><<<grid, 256, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C, ldc);
+    gemm_tile_kernel<float, true, false, 1><<<grid, 256, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C, ldc);
   else
-    gemm_tile_kernel<float, false, false><<<grid, 256, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C, ldc);
+    gemm_tile_kernel<float, false, false, 1><<<grid, 256, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C, ldc);
   return hipGetLastError();
 }
 
@@ -251,18 +280,27 @@ __device__ __forceinline__ void split3(float x, __bf16 *out) {
 // 16x16x32 MFMA step).  LDS holds the split planes [s][i][k] of A and
 // [s][j][k] of B (k contiguous: each lane reads 8 consecutive k); the next
 // stage's raw FP32 values are prefetched into registers during the MFMAs.
-template <int NS>
-__global__ void __launch_bounds__(256, 2)
+template <int NS, int TPW>
+__global__ void __launch_bounds__(256 * TPW, TPW == 1 ? 2 : 1)
     bgemm_tile_kernel(int M, int N, int K, const float *__restrict__ A, int64_t lda,
                       const float *__restrict__ B, int64_t ldb, float *__restrict__ C, int64_t ldc) {
   constexpr int BK = 32;
   constexpr int PK = BK + 8;  // padded k extent (bf16 elements)
-  __shared__ __bf16 As[NS][kTileM][PK];
-  __shared__ __bf16 Bs[NS][kTileN][PK];
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  __shared__ __bf16 As_all[TPW][NS][kTileM][PK];
+  __shared__ __bf16 Bs_all[TPW][NS][kTileN][PK];
+  const int h = threadIdx.x >> 8;
+  auto &As = As_all[h];
+  auto &Bs = Bs_all[h];
+  const int t = threadIdx.x & 255, lane = t & 63, wave = t >> 6;
   const int nbm = (M + kTileM - 1) / kTileM, nbn = (N + kTileN - 1) / kTileN;
-  int tm, tn;
-  tile_coords(nbm, nbn, tm, tn);
+  const int nbnp = (nbn + TPW - 1) / TPW;
+  const int ntile = nbm * nbnp;
+  // A capped grid loops over the tiles (same XCD grouping: gridDim.x is a
+  // multiple of 8 whenever it is below the tile count).
+  for (int id = blockIdx.x; id < ntile; id += gridDim.x) {
+  int tm, tnp;
+  tile_coords(id, nbm, nbnp, tm, tnp);
+  const int tn = tnp * TPW + h;
   const int bm = tm * kTileM, bn = tn * kTileN;
   const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
   // A: thread -> row t/2, k (t%2)*16 .. +16.  B: thread -> j 4*(t%32) .. +4,
@@ -393,16 +431,26 @@ __global__ void __launch_bounds__(256, 2)
         if (gi < M && gj < N) C[(int64_t)gi * ldc + gj] = cv[j][r] - acc[i][j][r];
       }
   }
+  }
 }
 
 hipError_t launch_bgemm_tile(lair_context *ctx, int ns, int M, int N, int K, const float *A,
                              int64_t lda, const float *B, int64_t ldb, float *C, int64_t ldc) {
   const int nbm = (M + kTileM - 1) / kTileM, nbn = (N + kTileN - 1) / kTileN;
+  if (ctx->bulk_wg_cap > 0) {
+    const int ntile = nbm * ((nbn + 1) / 2);
+    const unsigned grid = (unsigned)std::min(ntile, ctx->bulk_wg_cap);
+    if (ns == 3)
+      bgemm_tile_kernel<3, 2><<<grid, 512, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C, ldc);
+    else
+      bgemm_tile_kernel<1, 2><<<grid, 512, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C, ldc);
+    return hipGetLastError();
+  }
   const unsigned grid = (unsigned)(nbm * nbn);
   if (ns == 3)
-    bgemm_tile_kernel<3><<<grid, 256, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C, ldc);
+    bgemm_tile_kernel<3, 1><<<grid, 256, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C, ldc);
   else
-    bgemm_tile_kernel<1><<<grid, 256, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C, ldc);
+    bgemm_tile_kernel<1, 1><<<grid, 256, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C, ldc);
   return hipGetLastError();
 }
 

@@ -28,6 +28,12 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 // many polls instead of hanging when a peer never arrives.
 constexpr int kSpinLimit = 1 << 23;
 
+#ifndef LAIR_CALU_MARK
+#define LAIR_CALU_MARK(phase) \
+  do {                        \
+  } while (0)
+#endif
+
 template <typename T>
 __device__ __forceinline__ void st_agent(T *p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -45,6 +51,50 @@ __device__ __forceinline__ void better(double &bv, int &br, double ov, int orr) 
     bv = ov;
     br = orr;
   }
+}
+
+// Row segment I/O for the panel kernels: W consecutive elements of one row,
+// with 16-byte vector accesses when every row segment is 16-byte aligned
+// (vec, uniform) and whole; element accesses with zero fill otherwise.
+template <typename T, int W>
+__device__ __forceinline__ void load_row(T (&x)[W], const T *src, bool valid, int ncols, bool vec) {
+  constexpr int E = 16 / sizeof(T);
+  typedef T vec_t __attribute__((ext_vector_type(E)));
+  if (valid && vec && ncols >= W) {
+#pragma unroll
+    for (int q = 0; q < W / E; ++q) {
+      const vec_t u = reinterpret_cast<const vec_t *>(src)[q];
+#pragma unroll
+      for (int e = 0; e < E; ++e) x[q * E + e] = u[e];
+    }
+  } else {
+#pragma unroll
+    for (int w = 0; w < W; ++w) x[w] = (valid && w < ncols) ? src[w] : T(0);
+  }
+}
+
+template <typename T, int W>
+__device__ __forceinline__ void store_row(T *dst, const T (&x)[W], int ncols, bool vec) {
+  constexpr int E = 16 / sizeof(T);
+  typedef T vec_t __attribute__((ext_vector_type(E)));
+  if (vec && ncols >= W) {
+#pragma unroll
+    for (int q = 0; q < W / E; ++q) {
+      vec_t u;
+#pragma unroll
+      for (int e = 0; e < E; ++e) u[e] = x[q * E + e];
+      reinterpret_cast<vec_t *>(dst)[q] = u;
+    }
+  } else {
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+      if (w < ncols) dst[w] = x[w];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ bool rows_aligned(const T *a, int64_t lda, int col0) {
+  return ((lda * (int64_t)sizeof(T)) % 16 == 0) && ((reinterpret_cast<uintptr_t>(a + col0) & 15) == 0);
 }
 
 // ---------------------------------------------------------------- panel ---
@@ -374,12 +424,11 @@ __global__ void __launch_bounds__(NT) panel_coop_kernel(PanelArgs<T> pa) {
   const int base = pa.row0 + g * R;
 
   T v[RPT][W];
+  const bool vec = rows_aligned(pa.a, pa.lda, pa.col0);
 #pragma unroll
   for (int s = 0; s < RPT; ++s) {
     const int r = base + t + s * NT;
-    const T *src = pa.a + (int64_t)r * pa.lda + pa.col0;
-#pragma unroll
-    for (int w = 0; w < W; ++w) v[s][w] = (r < pa.m && w < pa.wcols) ? src[w] : T(0);
+    load_row<T, W>(v[s], pa.a + (int64_t)r * pa.lda + pa.col0, r < pa.m, pa.wcols, vec);
   }
 
   __shared__ double s_wv[2][NWAVE];
@@ -401,12 +450,7 @@ __global__ void __launch_bounds__(NT) panel_coop_kernel(PanelArgs<T> pa) {
 #pragma unroll
   for (int s = 0; s < RPT; ++s) {
     const int r = base + t + s * NT;
-    if (r < pa.m) {
-      T *dst = pa.a + (int64_t)r * pa.lda + pa.col0;
-#pragma unroll
-      for (int w = 0; w < W; ++w)
-        if (w < pa.wcols) dst[w] = v[s][w];
-    }
+    if (r < pa.m) store_row<T, W>(pa.a + (int64_t)r * pa.lda + pa.col0, v[s], pa.wcols, vec);
   }
   if (g != 0 || (pa.rest1 <= pa.col0 + pa.wcols && pa.left0 >= pa.col0)) return;
 
@@ -1184,6 +1228,463 @@ hipError_t launch_colmax(lair_context *ctx, int rows, int cols, const double *x,
   const int64_t total = (int64_t)rows * cols;
   if (total <= 0) return hipSuccess;
   colmax_kernel<<<(unsigned)((total + 255) / 256), 256, 0, ctx->stream>>>(rows, cols, x, ldx, bits);
+  return hipGetLastError();
+}
+
+
+// ------------------------------------------------------ tournament panel ---
+//
+// FP32 leaf of the mixed-precision factorization with tournament pivoting
+// (communication-avoiding LU, Grigori, Demmel and Xiang 2011): the 16 pivot
+// rows of a leaf are chosen by a reduction tree of small partial-pivoting
+// eliminations instead of one global argmax per column.
+//  1. every wave eliminates its own rows (K per lane) with partial pivoting
+//     and nominates the 16 rows it pivoted on;
+//  2. wave 0 repeats the elimination on the original values of the
+//     workgroup's 8 x 16 nominees and keeps 16;
+//  3. with P > 1 workgroups, each publishes its 16 nominees (tagged words,
+//     one round trip) and every workgroup eliminates the P x 16 finalists
+//     the same way, which fixes the pivot rows, their order, L11 and U11;
+//  4. the interchanges are expressed as a LAPACK pivot vector, the pivot rows
+//     take the diagonal positions and every other row gets L = a U11^-1 from
+//     its original values.
+// The result is an LU factorization of the row-permuted leaf, as with
+// partial pivoting, but the pivot order generally differs from it: this
+// kernel serves only the mixed-precision solver, whose answer is refined in
+// FP64 against the original matrix.
+
+struct CaluArgs {
+  float *a;
+  int64_t lda;
+  int row0, m, col0, wcols, npiv, P;
+  int64_t *ipiv;
+  int64_t *sing;
+  unsigned long long *rec;   // [kCaluMaxWG][kCaluRec] tagged words
+  unsigned long long *drec;  // [kCaluDiag] tagged words
+  uint32_t tag;
+  int32_t *err;
+  int rest1, left0;
+};
+
+// Partial-pivoting elimination of the rows held by one wave (K per lane,
+// ids[k] = row id or INT_MAX for an empty slot) over nsteps <= 16 columns.
+// Writes the chosen row ids in pivot order to out_ids, their slots
+// (lane + 64 k) to out_slot and, when out_rows is given, each pivot row as it
+// stood when chosen (multipliers left of the diagonal, U from it on).
+// Ties go to the lowest row id; a column without a nonzero candidate picks
+// the lowest remaining row and eliminates nothing.
+template <int K, int J>
+__device__ __forceinline__ void wave_gepp_step(float (&v)[K][kCaluW], const int (&ids)[K], bool (&used)[K],
+                                               int lane, float *s_p, int *out_ids, int *out_slot,
+                                               float (*out_rows)[kCaluW]) {
+  double bv = 0.0;
+  int bid = INT_MAX;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (!used[k]) {
+      const double x = fabs((double)v[k][J]);
+      if (bid == INT_MAX || x > bv || (x == bv && ids[k] < bid)) {
+        bv = x;
+        bid = ids[k];
+      }
+    }
+  }
+  double wv;
+  int wid;
+  wave_argmax(bv, bid, wv, wid);
+  if (wid == INT_MAX) {
+    if (lane == 0) {
+      out_ids[J] = INT_MAX;
+      out_slot[J] = -1;
+    }
+    return;
+  }
+  // The winning row (row ids are unique) copies itself to the wave's LDS row;
+  // LDS operations of one wave complete in order, so the other lanes read it
+  // back below.
+  int wslot = -1;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const bool win = !used[k] && ids[k] == wid;
+    if (win) {
+#pragma unroll
+      for (int w = 0; w < kCaluW; ++w) s_p[w] = v[k][w];
+      used[k] = true;
+      wslot = lane + 64 * k;
+    }
+  }
+  float p[kCaluW];
+#pragma unroll
+  for (int w = 0; w < kCaluW; ++w) p[w] = s_p[w];
+  if (wslot >= 0) {
+    out_ids[J] = wid;
+    out_slot[J] = wslot;
+    if (out_rows != nullptr) {
+#pragma unroll
+      for (int w = 0; w < kCaluW; ++w) out_rows[J][w] = p[w];
+    }
+  }
+  if (wv != 0.0) {
+    const float rp = 1.0f / p[J];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (!used[k]) {
+        const float l = v[k][J] * rp;
+        v[k][J] = l;
+#pragma unroll
+        for (int w = J + 1; w < kCaluW; ++w) v[k][w] = v[k][w] - l * p[w];
+      }
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <int K, int J = 0>
+__device__ __forceinline__ void wave_gepp_steps(float (&v)[K][kCaluW], const int (&ids)[K],
+                                                bool (&used)[K], int lane, int nsteps, float *s_p,
+                                                int *out_ids, int *out_slot,
+                                                float (*out_rows)[kCaluW]) {
+  if constexpr (J < kCaluW) {
+    if (J >= nsteps) return;
+    wave_gepp_step<K, J>(v, ids, used, lane, s_p, out_ids, out_slot, out_rows);
+    wave_gepp_steps<K, J + 1>(v, ids, used, lane, nsteps, s_p, out_ids, out_slot, out_rows);
+  }
+}
+
+// Partial-pivoting elimination of the rows held by one wave (K per lane,
+// ids[k] = row id or INT_MAX for an empty slot) over nsteps <= 16 columns.
+// Writes the chosen row ids in pivot order to out_ids, their slots
+// (lane + 64 k) to out_slot and, when out_rows is given, each pivot row as it
+// stood when chosen (multipliers left of the diagonal, U from it on).
+// Ties go to the lowest row id; a column without a nonzero candidate picks
+// the lowest remaining row and eliminates nothing.
+template <int K>
+__device__ __forceinline__ void wave_gepp(float (&v)[K][kCaluW], const int (&ids)[K], int lane,
+                                          int nsteps, float *s_p, int *out_ids, int *out_slot,
+                                          float (*out_rows)[kCaluW]) {
+  bool used[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) used[k] = ids[k] == INT_MAX;
+  wave_gepp_steps<K>(v, ids, used, lane, nsteps, s_p, out_ids, out_slot, out_rows);
+}
+
+// Spins on tagged words [0, total) of src until each carries tag, storing
+// the data halves to dst.  Thread t of nt takes words t, t + nt, ... (at most
+// kGatherMax each, all loads in flight together).  Returns false (for this
+// thread) when the bounded wait expires.
+constexpr int kGatherMax = 9;
+static_assert(kCaluMaxWG * kCaluRec <= kGatherMax * 512, "gather capacity");
+__device__ __forceinline__ bool gather_tagged(const unsigned long long *src, int total, uint32_t tag,
+                                              uint32_t *dst, int t, int nt, const int32_t *err) {
+  bool got[kGatherMax];
+#pragma unroll
+  for (int q = 0; q < kGatherMax; ++q) got[q] = t + q * nt >= total;
+  int spins = 0;
+  for (;;) {
+    bool all = true;
+#pragma unroll
+    for (int q = 0; q < kGatherMax; ++q) {
+      if (!got[q]) {
+        const int idx = t + q * nt;
+        const unsigned long long x = ld_agent(src + idx);
+        if ((uint32_t)(x >> 32) == tag) {
+          dst[idx] = (uint32_t)x;
+          got[q] = true;
+        } else {
+          all = false;
+        }
+      }
+    }
+    if (all) return true;
+    if ((++spins & 255) == 0 && (spins > kSpinLimit || ld_agent(err) != 0)) return false;
+  }
+}
+
+template <int K>
+__global__ void __launch_bounds__(512) calu_leaf_kernel(CaluArgs ca) {
+  constexpr int NT = 512, NWAVE = NT / 64, W = kCaluW;
+  constexpr int R = NT * K;
+  const int g = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int base = ca.row0 + g * R;
+  const int npiv = ca.npiv;
+
+  __shared__ int s_nid[NWAVE][W], s_nslot[NWAVE][W];
+  __shared__ float s_prow[NWAVE][W];               // each wave's pivot row
+  __shared__ float s_nval[NWAVE * W][W];           // nominees' original values
+  __shared__ int s_wid[W], s_wslot[W];             // workgroup nominees
+  __shared__ uint32_t s_fin[kCaluMaxWG * kCaluRec];  // finalists (ids, values)
+  __shared__ float s_diag[W][W];                   // original diagonal rows
+  __shared__ int s_fid[W], s_fslot[W];             // final pivot rows, in order
+  __shared__ float s_frow[W][W];                   // their L11 \ U11 rows
+  __shared__ int s_piv[W];
+  __shared__ int s_mrow[2 * W], s_msrc[2 * W], s_mn;
+  __shared__ int s_bail;
+
+  const bool vec = rows_aligned(ca.a, ca.lda, ca.col0);
+  LAIR_CALU_MARK(0);
+  // 1. Local elimination on a register copy of this thread's rows.
+  {
+    float v[K][W];
+    int ids[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int r = base + t + k * NT;
+      ids[k] = r < ca.m ? r : INT_MAX;
+      load_row<float, W>(v[k], ca.a + (int64_t)r * ca.lda + ca.col0, r < ca.m, ca.wcols, vec);
+    }
+    wave_gepp<K>(v, ids, lane, npiv, s_prow[wave], s_nid[wave], s_nslot[wave], nullptr);
+  }
+  if (lane == 0)
+    for (int j = npiv; j < W; ++j) s_nid[wave][j] = INT_MAX;
+  __syncthreads();
+  LAIR_CALU_MARK(1);
+  // Nominees' original values (still unmodified in memory).
+  for (int i = lane; i < W * W; i += 64) {
+    const int e = i / W, w = i % W;
+    const int id = s_nid[wave][e];
+    s_nval[wave * W + e][w] =
+        (id != INT_MAX && w < ca.wcols) ? ca.a[(int64_t)id * ca.lda + ca.col0 + w] : 0.0f;
+  }
+  if (g == 0 && wave == NWAVE - 1) {
+    for (int i = lane; i < W * W; i += 64) {
+      const int e = i / W, w = i % W;
+      const int r = ca.row0 + e;
+      s_diag[e][w] = (r < ca.m && w < ca.wcols) ? ca.a[(int64_t)r * ca.lda + ca.col0 + w] : 0.0f;
+    }
+  }
+  __syncthreads();
+  LAIR_CALU_MARK(2);
+  // 2. Workgroup round: 8 x 16 nominees, 2 per lane.
+  if (wave == 0) {
+    float v[2][W];
+    int ids[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int e = lane + 64 * k;
+      ids[k] = s_nid[e / W][e % W];
+#pragma unroll
+      for (int w = 0; w < W; ++w) v[k][w] = s_nval[e][w];
+    }
+    if (ca.P == 1) {
+      wave_gepp<2>(v, ids, lane, npiv, s_prow[0], s_fid, s_fslot, s_frow);
+    } else {
+      wave_gepp<2>(v, ids, lane, npiv, s_prow[0], s_wid, s_wslot, nullptr);
+      if (lane == 0)
+        for (int j = npiv; j < W; ++j) s_wid[j] = INT_MAX;
+    }
+  }
+  __syncthreads();
+  LAIR_CALU_MARK(3);
+  if (ca.P > 1) {
+    // 3. Publish the workgroup's nominees (and, from workgroup 0, the
+    // original diagonal rows); gather everyone's.
+    if (wave == 0) {
+      for (int i = lane; i < kCaluRec; i += 64) {
+        uint32_t data;
+        if (i < W) {
+          data = (uint32_t)s_wid[i];
+        } else {
+          const int e = (i - W) / W, w = (i - W) % W;
+          data = s_wid[e] == INT_MAX ? 0u : __float_as_uint(s_nval[s_wslot[e]][w]);
+        }
+        st_agent(ca.rec + (size_t)g * kCaluRec + i, tagged(ca.tag, data));
+      }
+    }
+    if (g == 0 && wave == 1) {
+      for (int i = lane; i < kCaluDiag; i += 64)
+        st_agent(ca.drec + i, tagged(ca.tag, __float_as_uint(s_diag[i / W][i % W])));
+    }
+    if (t == 0) s_bail = 0;
+    __syncthreads();
+    bool ok = gather_tagged(ca.rec, ca.P * kCaluRec, ca.tag, s_fin, t, NT, ca.err);
+    if (ok && g != 0)
+      ok = gather_tagged(ca.drec, kCaluDiag, ca.tag, reinterpret_cast<uint32_t *>(&s_diag[0][0]), t, NT,
+                         ca.err);
+    if (!ok) {
+      s_bail = 1;
+      st_agent(ca.err, 1);
+    }
+    __syncthreads();
+    if (s_bail) return;
+    LAIR_CALU_MARK(4);
+    // Final round on the P x 16 finalists.
+    if (wave == 0) {
+      const int nf = ca.P * W;
+      if (nf <= 128) {
+        float v[2][W];
+        int ids[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          const int e = lane + 64 * k;
+          const uint32_t *rw = s_fin + (e / W) * kCaluRec;
+          ids[k] = e < nf ? (int)rw[e % W] : INT_MAX;
+#pragma unroll
+          for (int w = 0; w < W; ++w) v[k][w] = e < nf ? __uint_as_float(rw[W + (e % W) * W + w]) : 0.0f;
+        }
+        wave_gepp<2>(v, ids, lane, npiv, s_prow[0], s_fid, s_fslot, s_frow);
+      } else {
+        float v[4][W];
+        int ids[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int e = lane + 64 * k;
+          const uint32_t *rw = s_fin + (e / W) * kCaluRec;
+          ids[k] = e < nf ? (int)rw[e % W] : INT_MAX;
+#pragma unroll
+          for (int w = 0; w < W; ++w) v[k][w] = e < nf ? __uint_as_float(rw[W + (e % W) * W + w]) : 0.0f;
+        }
+        wave_gepp<4>(v, ids, lane, npiv, s_prow[0], s_fid, s_fslot, s_frow);
+      }
+    }
+    __syncthreads();
+  }
+  LAIR_CALU_MARK(5);
+  // 4. Pivot vector and row map: position col0 + i receives row s_fid[i]
+  // (sequential interchanges, LAPACK convention).
+  if (t == 0) {
+    int nx = npiv;
+    for (int e = 0; e < npiv; ++e) {
+      s_mrow[e] = ca.col0 + e;
+      s_msrc[e] = ca.col0 + e;
+    }
+    for (int i = 0; i < npiv; ++i) {
+      const int want = s_fid[i];
+      int q = -1;  // current position of row `want`
+      for (int e = 0; e < nx; ++e)
+        if (s_msrc[e] == want) q = e;
+      if (q < 0) {  // untouched so far: it sits at its own position
+        q = nx++;
+        s_mrow[q] = want;
+        s_msrc[q] = want;
+      }
+      s_piv[i] = s_mrow[q];
+      const int tmp = s_msrc[i];
+      s_msrc[i] = s_msrc[q];
+      s_msrc[q] = tmp;
+    }
+    s_mn = nx;
+    if (g == 0) {
+      int64_t sg = -1;
+      for (int i = 0; i < npiv; ++i) {
+        ca.ipiv[ca.col0 + i] = s_piv[i];
+        if (s_frow[i][i] == 0.0f) sg = ca.col0 + i;
+      }
+      if (sg >= 0) *ca.sing = sg;
+    }
+  }
+  __syncthreads();
+  // 5. Final values of this workgroup's rows: diagonal positions take the
+  // pivot rows' L11 \ U11; every other position gets L = a U11^-1 from the
+  // original values of the row it now holds.
+  const int nent = s_mn;
+#pragma unroll 1
+  for (int k = 0; k < K; ++k) {
+    const int r = base + t + k * NT;
+    if (r >= ca.m) continue;
+    float x[W];
+    if (r < ca.col0 + npiv) {
+#pragma unroll
+      for (int w = 0; w < W; ++w) x[w] = s_frow[r - ca.col0][w];
+    } else {
+      int src = r;
+      for (int e = npiv; e < nent; ++e)
+        if (s_mrow[e] == r) src = s_msrc[e];
+      if (src != r) {  // a displaced diagonal row
+#pragma unroll
+        for (int w = 0; w < W; ++w) x[w] = s_diag[src - ca.col0][w];
+      } else {
+        load_row<float, W>(x, ca.a + (int64_t)r * ca.lda + ca.col0, true, ca.wcols, vec);
+      }
+      // A zero pivot leaves its column unscaled and out of the later
+      // updates, as getrf.rs:72-73 does for an all-zero column.
+      bool live[W];
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        live[j] = false;
+        if (j < npiv) {
+          float acc = x[j];
+#pragma unroll
+          for (int i = 0; i < j; ++i)
+            if (live[i]) acc = acc - x[i] * s_frow[i][j];
+          const float ujj = s_frow[j][j];
+          live[j] = ujj != 0.0f;
+          x[j] = live[j] ? acc / ujj : acc;
+        }
+      }
+      // Columns past npiv (short leaves): the U part of the update.
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        if (w >= npiv && w < ca.wcols) {
+          float acc = x[w];
+#pragma unroll
+          for (int i = 0; i < W; ++i)
+            if (i < npiv && live[i]) acc = acc - x[i] * s_frow[i][w];
+          x[w] = acc;
+        }
+      }
+    }
+    store_row<float, W>(ca.a + (int64_t)r * ca.lda + ca.col0, x, ca.wcols, vec);
+  }
+  LAIR_CALU_MARK(6);
+  if (g != 0 || (ca.rest1 <= ca.col0 + ca.wcols && ca.left0 >= ca.col0)) return;
+  // 6. Rest of the panel (workgroup 0; see panel_coop_kernel): interchanges
+  // on [left0, col0) and [col0 + wcols, rest1), U12 = L11^-1 A12 on the latter.
+  __shared__ float s_buf[2 * W][64];
+  const int c0 = ca.col0 + ca.wcols;
+  const int ql = t & 63, eg = t >> 6;
+  const int nleft = ca.left0 < ca.col0 ? ca.col0 - ca.left0 : 0;
+  const int ntot = nleft + (ca.rest1 > c0 ? ca.rest1 - c0 : 0);
+  for (int cb = 0; cb < ntot; cb += 64) {
+    const int qi = cb + ql;
+    const bool act = qi < ntot;
+    const int q = qi < nleft ? ca.left0 + qi : c0 + (qi - nleft);
+    const bool solve = qi >= nleft;
+    for (int e = eg; e < nent; e += NWAVE)
+      if (act) s_buf[e][ql] = ca.a[(int64_t)s_msrc[e] * ca.lda + q];
+    __syncthreads();
+    if (eg == 0 && act && solve) {
+      for (int i = 1; i < npiv; ++i) {
+        float acc = s_buf[i][ql];
+        for (int p = 0; p < i; ++p) acc -= s_frow[i][p] * s_buf[p][ql];
+        s_buf[i][ql] = acc;
+      }
+    }
+    __syncthreads();
+    for (int e = eg; e < nent; e += NWAVE)
+      if (act && ((solve && e < npiv) || s_msrc[e] != s_mrow[e]))
+        ca.a[(int64_t)s_mrow[e] * ca.lda + q] = s_buf[e][ql];
+    __syncthreads();
+  }
+}
+
+hipError_t launch_panel_calu(lair_context *ctx, float *a, int64_t lda, int c, int m, int wcols,
+                             int64_t *ipiv, int rest1, int left0) {
+  const int rows = m - c;
+  if (rows <= 0 || wcols <= 0) return hipSuccess;
+  if (wcols > kCaluW) return hipErrorInvalidValue;
+  CaluArgs ca;
+  ca.a = a;
+  ca.lda = lda;
+  ca.row0 = c;
+  ca.m = m;
+  ca.col0 = c;
+  ca.wcols = wcols;
+  ca.npiv = wcols < rows ? wcols : rows;
+  ca.ipiv = ipiv;
+  ca.sing = ctx->sing;
+  ca.rec = ctx->crec;
+  ca.drec = ctx->crec + (size_t)kCaluMaxWG * kCaluRec;
+  ca.tag = ctx->seq + 1u;
+  ca.err = ctx->err;
+  ca.rest1 = rest1;
+  ca.left0 = left0;
+  constexpr int K = 4, R = 512 * K;
+  ca.P = (rows + R - 1) / R;
+  if (ca.P > kCaluMaxWG) return hipErrorInvalidValue;
+  const int slot = prof_begin(ctx, LAIR_KCLASS_PANEL);
+  calu_leaf_kernel<K><<<ca.P, 512, 0, ctx->stream>>>(ca);
+  prof_end(ctx, slot, LAIR_KCLASS_PANEL, (double)ca.npiv * rows * wcols, 8.0 * rows * wcols);
+  ctx->seq += (uint32_t)kLeafW;
   return hipGetLastError();
 }
 

@@ -21,6 +21,15 @@ constexpr int kLeafW = 16;
 constexpr int kExactOrderMax = 128;
 // Maximum number of workgroups in a cooperative panel launch.
 constexpr int kMaxPanelWG = 64;
+// ... and in a tournament-pivoting leaf (2048 rows each), whose records hold
+// a workgroup's 16 nominee ids and values, plus the 16 original diagonal rows.
+constexpr int kCaluMaxWG = 16;
+constexpr int kCaluW = 16;
+constexpr int kCaluRec = kCaluW + kCaluW * kCaluW;
+constexpr int kCaluDiag = kCaluW * kCaluW;
+constexpr int kCaluWords = kCaluMaxWG * kCaluRec + kCaluDiag;
+// CUs kept free of the look-ahead stream's bulk updates for the critical path.
+constexpr int kReservedCUs = 32;
 // Row interchanges of one panel touch at most 2 * (panel width) rows.
 constexpr int kMaxSwapRows = 512;
 // Largest n whose row permutation is composed in LDS.
@@ -71,6 +80,7 @@ struct lair_context {
   // Tagged exchange records: [2][kMaxPanelWG][64] and [2][64] 64-bit words.
   unsigned long long *rec = nullptr;
   unsigned long long *drec = nullptr;
+  unsigned long long *crec = nullptr;  // tournament leaf records (tagged words)
   uint32_t seq = 0;            // tag base of the next panel launch
   int panel_xcd_stride = 1;    // launched slots per panel workgroup (see launch_panel)
   int32_t *err = nullptr;      // set when a bounded wait expires
@@ -86,6 +96,8 @@ struct lair_context {
   // critical path (next panel's columns, next panel) continues on stream.
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_panel = nullptr, ev_bulk = nullptr;
+  int num_cu = 256;
+  int bulk_wg_cap = 0;  // > 0: tile GEMMs launch at most this many workgroups
 
   // Grow-only workspaces.
   lair::Buffer ws_a32, ws_lu64, ws_ipiv, ws_x32, ws_r, ws_tmp, ws_rhs;
@@ -107,10 +119,18 @@ namespace lair {
 
 // Redirects the context's launches to stream2 (and its row-map buffers) for
 // the lifetime of the object.
+// Its tile GEMMs run on at most (CUs - kReservedCUs) workgroups, one per CU,
+// so the reserved CUs stay free for the critical path.
 struct SideStream {
   lair_context *ctx;
-  explicit SideStream(lair_context *c) : ctx(c) { swap(); }
-  ~SideStream() { swap(); }
+  explicit SideStream(lair_context *c) : ctx(c) {
+    swap();
+    ctx->bulk_wg_cap = (ctx->num_cu - kReservedCUs) / 8 * 8;
+  }
+  ~SideStream() {
+    ctx->bulk_wg_cap = 0;
+    swap();
+  }
   void swap() {
     std::swap(ctx->stream, ctx->stream2);
     std::swap(ctx->swap_rows, ctx->swap_rows_alt);
@@ -130,6 +150,9 @@ hipError_t ensure(Buffer &b, size_t bytes);
 template <typename T>
 hipError_t launch_panel(lair_context *ctx, T *a, int64_t lda, int row0, int m, int col0, int wcols,
                         int64_t *ipiv, int rest1, int left0);
+// Tournament-pivoting FP32 leaf (mixed-precision factorization only).
+hipError_t launch_panel_calu(lair_context *ctx, float *a, int64_t lda, int c, int m, int wcols,
+                             int64_t *ipiv, int rest1, int left0);
 
 // Row interchanges ipiv[k0..k1) (forward, or reverse when rev) applied to the
 // columns [c0, c1) minus [s0, s1) of a (row-major, lda).
