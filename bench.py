@@ -49,7 +49,26 @@ def gen_lib():
     lib.lair_gen_kappa.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_double,
                                    ctypes.c_uint64]
     lib.lair_gen_kappa.restype = ctypes.c_int
+    lib.lair_gen_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
+    lib.lair_gen_copy.restype = ctypes.c_int
     return lib
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trailing_update_traffic.json")
+
+
+def measured_traffic(head):
+    """HBM bytes per launch of the headline trailing update, from the committed
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by
+    the same counters' reading for a copy kernel of known size); None when no
+    summary for this variant exists."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    entry = t.get("bf16x6" if head.startswith("bf16x6") else "fp32")
+    return entry.get("hbm_bytes_per_launch") if entry else None
 
 
 def prof_read(ctx, kname):
@@ -88,7 +107,14 @@ def main():
     ap.add_argument("--nrhs", type=int, default=1)
     ap.add_argument("--kappa", type=float, default=1e6)
     ap.add_argument("--precisions", default="bf16x6+tp,fp32+tp,bf16x6,fp32")
-    ap.add_argument("--cpu-sample-n", type=int, default=2048)
+    ap.add_argument("--cpu-sample-n", type=int, default=5120)
+    ap.add_argument("--only-mixed", action="store_true",
+                    help="skip the FP64 solve (counter-collection runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a 256 MiB copy kernel (known bytes for counter calibration)")
+    ap.add_argument("--gaussian", action="store_true",
+                    help="A ~ N(0,1) from the library's own generator kernel instead of the kappa=1e6 "
+                         "construction (same factorization shapes; for counter-collection runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -108,7 +134,10 @@ def main():
     db = L.DeviceArray(n, nrhs)
     dx = L.DeviceArray(n, nrhs)
     t = time.perf_counter()
-    rc = gl.lair_gen_kappa(da.ptr, n, ld, args.kappa, 16384)
+    if args.gaussian:
+        rc = gl.lair_gen_gaussian(da.ptr, n, n, ld, 16384)
+    else:
+        rc = gl.lair_gen_kappa(da.ptr, n, ld, args.kappa, 16384)
     rc |= gl.lair_gen_gaussian(db.ptr, n, nrhs, nrhs, 16385)
     if rc:
         raise RuntimeError(f"input generation failed ({rc})")
@@ -122,6 +151,15 @@ def main():
     h2d_ms = (time.perf_counter() - t) * 1e3
     del host
 
+    if args.copy_calibration:
+        cb = 256 << 20
+        src, dst = L.DeviceArray(cb // 8, 1), L.DeviceArray(cb // 8, 1)
+        if gl.lair_gen_copy(dst.ptr, src.ptr, cb):
+            raise RuntimeError("calibration copy failed")
+        log(f"calibration copy of {cb} bytes done")
+        src.free()
+        dst.free()
+
     def berr():
         v = ctypes.c_double()
         L.check(lib.lair_dbackward_error(ctx.h, n, nrhs, da.ptr, ld, db.ptr, nrhs, dx.ptr, nrhs,
@@ -132,16 +170,18 @@ def main():
     def fp64_step():
         L.check(lib.lair_dgesv_device(ctx.h, n, nrhs, da.ptr, ld, db.ptr, nrhs, dx.ptr, nrhs), "dgesv")
 
-    for _ in range(W):
-        fp64_step()
-    ctx.synchronize()
-    t = time.perf_counter()
-    for _ in range(K):
-        fp64_step()
-    ctx.synchronize()
-    fp64_ms = (time.perf_counter() - t) * 1e3 / K
-    fp64_berr = berr()
-    log(f"fp64 LU: {fp64_ms:.2f} ms/solve, backward error {fp64_berr:.2e}")
+    fp64_ms, fp64_berr = None, None
+    if not args.only_mixed:
+        for _ in range(W):
+            fp64_step()
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(K):
+            fp64_step()
+        ctx.synchronize()
+        fp64_ms = (time.perf_counter() - t) * 1e3 / K
+        fp64_berr = berr()
+        log(f"fp64 LU: {fp64_ms:.2f} ms/solve, backward error {fp64_berr:.2e}")
 
     results = {}
     for name in [p for p in args.precisions.split(",") if p]:
@@ -201,7 +241,7 @@ def main():
         achieved = g_fl / (g_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": mfma_peak, "unit": "TFLOP/s",
                 "frac": round(achieved / mfma_peak, 4)}
-    roof.update({"traffic": None, "kernel": f"trailing update ({head})",
+    roof.update({"traffic": measured_traffic(head), "kernel": f"trailing update ({head})",
                  "launches_per_step": g_cnt, "algorithmic_flops_per_step": g_fl,
                  "algorithmic_bytes_per_step": g_by, "ms_per_step": round(g_ms, 3)})
 
@@ -228,8 +268,8 @@ def main():
         "config": {
             "workload": f"dgesv n={n} nrhs={nrhs} kappa={args.kappa:g} (Haar U,V; log-spaced s)",
             "n": n, "nrhs": nrhs, "kappa": args.kappa, "precision": head,
-            "fp64_ms": round(fp64_ms, 3),
-            "speedup_vs_fp64": round(fp64_ms / hr["ms"], 3),
+            "fp64_ms": round(fp64_ms, 3) if fp64_ms else None,
+            "speedup_vs_fp64": round(fp64_ms / hr["ms"], 3) if fp64_ms else None,
             "backward_error": hr["backward_error"],
             "fp64_backward_error": fp64_berr,
             "tolerance": n * EPS,

@@ -71,7 +71,23 @@ int haar_in_place(rocblas_handle h, double *g, int64_t n, int64_t ld, double *ta
 
 }  // namespace
 
+// Plain 16-byte copy of `n16` 16-byte words: the known-size reference that
+// calibrates HBM byte counters (FETCH_SIZE / WRITE_SIZE) against the bytes a
+// kernel must move.
+__global__ void copy16_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, int64_t n16) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 extern "C" {
+
+// dst <- src, `bytes` a multiple of 16 (device pointers, 16-byte aligned).
+int lair_gen_copy(void *dst, const void *src, int64_t bytes) {
+  if (bytes <= 0 || bytes % 16 != 0) return 1;
+  copy16_kernel<<<2048, 256>>>(static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), bytes / 16);
+  if (hipDeviceSynchronize() != hipSuccess) return 3;
+  return 0;
+}
 
 int lair_gen_gaussian(double *a, int64_t rows, int64_t cols, int64_t ld, uint64_t seed) {
   const int64_t total = rows * cols;

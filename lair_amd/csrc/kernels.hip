@@ -10,6 +10,8 @@
 //  * FP64 residual, norms and conversions for iterative refinement.
 #include <climits>
 
+#include <algorithm>
+
 #include "lair_internal.h"
 
 // Phase marker inside the panel kernel: expands to nothing in the library;
@@ -951,7 +953,7 @@ hipError_t launch_gemm_minus<double>(lair_context *ctx, int mode, int M, int N, 
                                      int64_t ldb, double *C, int64_t ldc, int kclass) {
   if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
   (void)mode;
-  if (M >= 256 && N >= 128 && K >= 32) {
+  if (M >= 256 && N >= 32 && K >= 32) {
     const int slot = prof_begin(ctx, kclass);
     hipError_t e = launch_dgemm_tile(ctx, M, N, K, A, lda, transA, B, ldb, C, ldc);
     prof_end(ctx, slot, kclass, 2.0 * M * N * K,
@@ -974,7 +976,7 @@ hipError_t launch_gemm_minus<float>(lair_context *ctx, int mode, int M, int N, i
                                     const float *A, int64_t lda, bool transA, const float *B,
                                     int64_t ldb, float *C, int64_t ldc, int kclass) {
   if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
-  if (M >= 256 && N >= 128 && K >= 32 && !(transA && mode != kGemmF32)) {
+  if (M >= 256 && N >= 32 && K >= 32 && !(transA && mode != kGemmF32)) {
     const int slot = prof_begin(ctx, kclass);
     hipError_t e;
     if (mode == kGemmF32)
@@ -1214,20 +1216,35 @@ hipError_t launch_residual(lair_context *ctx, int n, int nrhs, const double *a, 
   return launch_gemm_minus<double>(ctx, kGemmF64, n, nrhs, n, a, lda, false, x, ldx, r, ldr);
 }
 
-__global__ void colmax_kernel(int rows, int cols, const double *x, int64_t ldx,
-                              unsigned long long *bits) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t total = (int64_t)rows * cols;
-  if (i >= total) return;
-  const int r = (int)(i / cols), c = (int)(i % cols);
-  atomic_max_nonneg(bits + c, fabs(x[(int64_t)r * ldx + c]));
+// Column maxima of |x| as ordered bit patterns (NaN above +inf): a
+// grid-stride pass per column, a workgroup reduction of the bit patterns in
+// LDS, then one atomic per workgroup and column.
+__global__ void __launch_bounds__(256) colmax_kernel(int rows, int cols, const double *x, int64_t ldx,
+                                                     unsigned long long *bits) {
+  __shared__ unsigned long long red[256];
+  const int c = blockIdx.y;
+  unsigned long long m = 0ull;
+  for (int r = blockIdx.x * 256 + threadIdx.x; r < rows; r += gridDim.x * 256) {
+    const unsigned long long v = (unsigned long long)__double_as_longlong(fabs(x[(int64_t)r * ldx + c]));
+    m = v > m ? v : m;
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      const unsigned long long o = red[threadIdx.x + off];
+      if (o > red[threadIdx.x]) red[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(bits + c, red[0]);
 }
 
 hipError_t launch_colmax(lair_context *ctx, int rows, int cols, const double *x, int64_t ldx,
                          unsigned long long *bits) {
-  const int64_t total = (int64_t)rows * cols;
-  if (total <= 0) return hipSuccess;
-  colmax_kernel<<<(unsigned)((total + 255) / 256), 256, 0, ctx->stream>>>(rows, cols, x, ldx, bits);
+  if (rows <= 0 || cols <= 0) return hipSuccess;
+  const int gx = std::min((rows + 255) / 256, 128);
+  colmax_kernel<<<dim3(gx, cols), 256, 0, ctx->stream>>>(rows, cols, x, ldx, bits);
   return hipGetLastError();
 }
 
