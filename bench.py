@@ -51,7 +51,130 @@ def gen_lib():
     lib.lair_gen_kappa.restype = ctypes.c_int
     lib.lair_gen_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
     lib.lair_gen_copy.restype = ctypes.c_int
+    lib.lair_gen_batch_shift.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    lib.lair_gen_batch_shift.restype = ctypes.c_int
+    lib.lair_gen_to_f32.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
+    lib.lair_gen_to_f32.restype = ctypes.c_int
+    lib.lair_gen_shift_diag.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_double]
+    lib.lair_gen_shift_diag.restype = ctypes.c_int
     return lib
+
+
+def bench_batched(ctx, gl, dtype, steps, count=20000, seed=4):
+    """C4: `count` systems, n ~ U{8..256}, A ~ N(0,1) + n I, nrhs ~ U{1,2,4},
+    solved in one graph-captured persistent launch (plan autotuned once).
+    Inputs are regenerated on the device before every timed launch (untimed)."""
+    import lair_amd as L
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(8, 257, count).astype(np.int32)
+    rs = rng.choice(np.array([1, 2, 4], np.int32), count)
+    offs = np.concatenate([[0], np.cumsum(ns.astype(np.int64) ** 2)[:-1]]).astype(np.int64)
+    na, nb_ = int((ns.astype(np.int64) ** 2).sum()), int((ns.astype(np.int64) * rs).sum())
+    w = np.dtype(dtype).itemsize
+    d_ns, d_offs = L.DeviceArray.from_host(ns.reshape(-1, 1)), L.DeviceArray.from_host(offs.reshape(-1, 1))
+    a64, b64 = L.DeviceArray(na, 1), L.DeviceArray(nb_, 1)
+    da, db = L.DeviceArray(na, 1, dtype), L.DeviceArray(nb_, 1, dtype)
+    dp, di = L.DeviceArray(int(ns.sum()), 1, np.int64), L.DeviceArray(count, 1, np.int32)
+
+    def regen():
+        rc = gl.lair_gen_gaussian(a64.ptr, na, 1, 1, seed)
+        rc |= gl.lair_gen_batch_shift(a64.ptr, d_ns.ptr, d_offs.ptr, count)
+        rc |= gl.lair_gen_gaussian(b64.ptr, nb_, 1, 1, seed + 1)
+        if dtype == np.float64:
+            ctx.lib.lair_memcpy_d2d(da.ptr, a64.ptr, na * 8)
+            ctx.lib.lair_memcpy_d2d(db.ptr, b64.ptr, nb_ * 8)
+        else:
+            rc |= gl.lair_gen_to_f32(a64.ptr, da.ptr, na)
+            rc |= gl.lair_gen_to_f32(b64.ptr, db.ptr, nb_)
+        if rc:
+            raise RuntimeError("batched input generation failed")
+
+    regen()
+    plan = L.BatchedPlan(ctx, ns, rs, da, db, dp, di, precision=0 if dtype == np.float64 else 1)
+    times = []
+    for _ in range(steps + 1):
+        regen()
+        ctx.synchronize()
+        t = time.perf_counter()
+        plan.run()
+        ctx.synchronize()
+        times.append(time.perf_counter() - t)
+    ms = float(np.mean(times[1:])) * 1e3
+    info = di.download().ravel()
+    # Spot check: backward error of three systems against their inputs.
+    def fetch(arr, off, cnt, dt):
+        h = np.empty(cnt, dt)
+        ctx.lib.lair_memcpy_d2h(ctypes.c_void_p(h.ctypes.data), ctypes.c_void_p(arr.ptr.value + off * h.itemsize),
+                                h.nbytes)
+        return h
+    boffs = np.concatenate([[0], np.cumsum(ns.astype(np.int64) * rs)[:-1]])
+    bes = []
+    for i in (0, count // 2, count - 1):
+        n, r = int(ns[i]), int(rs[i])
+        A = fetch(a64, int(offs[i]), n * n, np.float64).reshape(n, n)
+        B = fetch(b64, int(boffs[i]), n * r, np.float64).reshape(n, r)
+        if dtype == np.float32:
+            A, B = A.astype(np.float32).astype(np.float64), B.astype(np.float32).astype(np.float64)
+        X = fetch(db, int(boffs[i]), n * r, dtype).reshape(n, r).astype(np.float64)
+        bes.append(float(np.abs(B - A @ X).max() / (np.abs(A).sum(1).max() * np.abs(X).max())))
+    cfg = plan.config()
+    plan.close()
+    nf = ns.astype(np.float64)
+    flops = float((2 * nf ** 3 / 3 + 2 * nf ** 2 * rs).sum())
+    byts = float((w * (2 * nf ** 2 + 2 * nf * rs)).sum())
+    for d in (d_ns, d_offs, a64, b64, da, db, dp, di):
+        d.free()
+    return {"ms": round(ms, 3), "systems": count, "systems_per_s": round(count / (ms * 1e-3), 1),
+            "singular": int((info >= 0).sum()), "sample_backward_error": max(bes),
+            "algorithmic_bytes": byts, "algorithmic_flops": flops,
+            "hbm_gbs": round(byts / (ms * 1e-3) / 1e9, 1),
+            "hbm_frac": round(byts / (ms * 1e-3) / 1e9 / PEAK["hbm_gbs"], 4),
+            "plan": cfg}
+
+
+def bench_adjoint(ctx, gl, steps, n=4096, nrhs=64):
+    """C5: A ~ N(0,1) + 4 sqrt(n) I, B and XBAR ~ N(0,1); times
+    lair_dsolve_backward_device (BBAR = A^-T XBAR by a transposed getrs on the
+    stored factors, ABAR = -BBAR X^T by the FP64 MFMA GEMM)."""
+    import lair_amd as L
+    ld = n + 64
+    da, db, dxb = L.DeviceArray(n, n, ld=ld), L.DeviceArray(n, nrhs), L.DeviceArray(n, nrhs)
+    rc = gl.lair_gen_gaussian(da.ptr, n, n, ld, 4096)
+    rc |= gl.lair_gen_shift_diag(da.ptr, n, ld, 4.0 * np.sqrt(n))
+    rc |= gl.lair_gen_gaussian(db.ptr, n, nrhs, nrhs, 4097)
+    rc |= gl.lair_gen_gaussian(dxb.ptr, n, nrhs, nrhs, 4098)
+    if rc:
+        raise RuntimeError("adjoint input generation failed")
+    lib = ctx.lib
+    dlu = L.DeviceArray(n, n, ld=ld)
+    lib.lair_memcpy_d2d(dlu.ptr, da.ptr, n * ld * 8)
+    dp = L.DeviceArray(n, 1, np.int64)
+    sing = ctypes.c_int64(-1)
+    L.check(lib.lair_dgetrf_device(ctx.h, n, n, dlu.ptr, ld, dp.ptr, 0, ctypes.byref(sing)), "getrf")
+    dx = L.DeviceArray(n, nrhs)
+    lib.lair_memcpy_d2d(dx.ptr, db.ptr, n * nrhs * 8)
+    L.check(lib.lair_dgetrs_device(ctx.h, n, nrhs, dlu.ptr, ld, dp.ptr, dx.ptr, nrhs, 0), "getrs")
+    dbb, dab = L.DeviceArray(n, nrhs), L.DeviceArray(n, n)
+    times = []
+    for _ in range(steps + 1):
+        ctx.synchronize()
+        t = time.perf_counter()
+        L.check(lib.lair_dsolve_backward_device(ctx.h, n, nrhs, dlu.ptr, ld, dp.ptr, dx.ptr, nrhs, dxb.ptr,
+                                                nrhs, dbb.ptr, nrhs, dab.ptr, n), "solve_backward")
+        ctx.synchronize()
+        times.append(time.perf_counter() - t)
+    ms = float(np.mean(times[1:])) * 1e3
+    # Closed-form check on the host: A^T BBAR = XBAR.
+    a = da.download()
+    bbar, xbar = dbb.download(), dxb.download()
+    rel = float(np.abs(a.T @ bbar - xbar).max() / (np.abs(a).sum(0).max() * np.abs(bbar).max()))
+    byts = 2.0 * n * n * 8
+    for d in (da, db, dxb, dlu, dp, dx, dbb, dab):
+        d.free()
+    return {"ms": round(ms, 3), "n": n, "nrhs": nrhs, "bbar_backward_error": rel,
+            "algorithmic_flops": 2.0 * n * n * nrhs * 2, "algorithmic_bytes": byts,
+            "hbm_gbs": round(byts / (ms * 1e-3) / 1e9, 1),
+            "hbm_frac": round(byts / (ms * 1e-3) / 1e9 / PEAK["hbm_gbs"], 4)}
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trailing_update_traffic.json")
@@ -112,6 +235,8 @@ def main():
                     help="skip the FP64 solve (counter-collection runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 256 MiB copy kernel (known bytes for counter calibration)")
+    ap.add_argument("--no-side", action="store_true",
+                    help="skip the batched (C4) and adjoint (C5) measurements")
     ap.add_argument("--gaussian", action="store_true",
                     help="A ~ N(0,1) from the library's own generator kernel instead of the kappa=1e6 "
                          "construction (same factorization shapes; for counter-collection runs)")
@@ -245,6 +370,17 @@ def main():
                  "launches_per_step": g_cnt, "algorithmic_flops_per_step": g_fl,
                  "algorithmic_bytes_per_step": g_by, "ms_per_step": round(g_ms, 3)})
 
+    side = {}
+    if not args.no_side and not args.only_mixed:
+        for d in (da, db, dx):
+            d.free()
+        for name, dt in (("fp64", np.float64), ("fp32", np.float32)):
+            side[f"batched_{name}"] = bench_batched(ctx, gl, dt, max(K, 3))
+            log(f"batched {name}: {side[f'batched_{name}']['ms']:.3f} ms for 20000 systems "
+                f"({side[f'batched_{name}']['hbm_gbs']} GB/s algorithmic)")
+        side["adjoint"] = bench_adjoint(ctx, gl, max(K, 3))
+        log(f"adjoint n=4096 nrhs=64: {side['adjoint']['ms']:.3f} ms")
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -279,13 +415,17 @@ def main():
                          for k, v in results.items()},
             "h2d_ms_A": round(h2d_ms, 2),
             "breakdown_ms": breakdown,
+            "batched_c4": side.get("batched_fp64"),
+            "batched_c4_fp32": side.get("batched_fp32"),
+            "adjoint_c5": side.get("adjoint"),
         },
         "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
-    for d in (da, db, dx):
-        d.free()
+    if not side:
+        for d in (da, db, dx):
+            d.free()
     ctx.close()
 
 

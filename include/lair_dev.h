@@ -69,7 +69,8 @@ void *lair_context_stream(lair_context *ctx);
 /* Human-readable message for the last error seen on this thread. */
 const char *lair_last_error(void);
 
-/* Device memory helpers, so callers need no other GPU runtime. */
+/* Device memory helpers, so callers need no other GPU runtime.  The copies
+ * block until the data has arrived. */
 lair_status lair_malloc(void **dptr, uint64_t bytes);
 lair_status lair_free(void *dptr);
 lair_status lair_memcpy_h2d(void *dst, const void *src, uint64_t bytes);

@@ -571,7 +571,11 @@ lair_status lair_memcpy_d2h(void *dst, const void *src, uint64_t bytes) {
   return LAIR_OK;
 }
 lair_status lair_memcpy_d2d(void *dst, const void *src, uint64_t bytes) {
+  // A device-to-device hipMemcpy may return before the copy completes; the
+  // copies here are blocking like the other two, so work a caller then queues
+  // on a context's (non-blocking) stream sees the copied data.
   LAIR_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  LAIR_HIP(hipDeviceSynchronize());
   return LAIR_OK;
 }
 

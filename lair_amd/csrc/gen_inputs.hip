@@ -79,7 +79,48 @@ __global__ void copy16_kernel(const uint4 *__restrict__ src, uint4 *__restrict__
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
 }
 
+// Adds scale * n_i to the diagonal of every packed system (order ns[i] at
+// element offset offs[i], row-major): one workgroup per system.
+__global__ void batch_shift_kernel(double *a, const int *ns, const int64_t *offs, int count) {
+  const int s = blockIdx.x;
+  if (s >= count) return;
+  const int n = ns[s];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) a[offs[s] + (int64_t)i * n + i] += (double)n;
+}
+
+__global__ void to_f32_kernel(const double *src, float *dst, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = (float)src[i];
+}
+
+__global__ void shift_diag_kernel(double *a, int64_t n, int64_t ld, double s) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i * ld + i] += s;
+}
+
 extern "C" {
+
+// Packed batch of `count` systems: a already holds N(0,1) entries; adds n_i on
+// each system's diagonal (ns, offs: device arrays).
+int lair_gen_batch_shift(double *a, const int *ns, const int64_t *offs, int count) {
+  if (count <= 0) return 0;
+  batch_shift_kernel<<<count, 256>>>(a, ns, offs, count);
+  return hipDeviceSynchronize() == hipSuccess ? 0 : 3;
+}
+
+// dst[i] = (float) src[i], i < n (device pointers).
+int lair_gen_to_f32(const double *src, float *dst, int64_t n) {
+  if (n <= 0) return 0;
+  to_f32_kernel<<<2048, 256>>>(src, dst, n);
+  return hipDeviceSynchronize() == hipSuccess ? 0 : 3;
+}
+
+// a[i][i] += s for i < n (row-major, leading dimension ld).
+int lair_gen_shift_diag(double *a, int64_t n, int64_t ld, double s) {
+  if (n <= 0) return 0;
+  shift_diag_kernel<<<(unsigned)((n + 255) / 256), 256>>>(a, n, ld, s);
+  return hipDeviceSynchronize() == hipSuccess ? 0 : 3;
+}
 
 // dst <- src, `bytes` a multiple of 16 (device pointers, 16-byte aligned).
 int lair_gen_copy(void *dst, const void *src, int64_t bytes) {
