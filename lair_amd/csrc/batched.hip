@@ -102,10 +102,12 @@ struct BatchSolver {
   int t, lane, wave;
   T *sL;      // [kBatchMaxN][NB]: L11 and L21 of the current panel
   T *sU;      // [NB][kBatchLDU]: U12 (and the rows of B) of the current panel
-  T *sprow;   // [2][NB]: pivot row and diagonal row
+  T *sprow;   // [2][2][NB]: pivot row and diagonal row, by column parity
   int *spiv;  // [NB]
   int *srows, *ssrc, *scnt;  // row map of the panel's interchanges
   int *sinfo;
+  double *s_wv;  // [2][4] wave maxima (workgroup panel), by column parity
+  int *s_wr;     // [2][4] their rows
 
   // Row map of the interchanges spiv[0..w) (absolute rows >= k0): rows[e]
   // receives the contents of src[e].  Lane 0 of wave 0 composes it.
@@ -247,13 +249,111 @@ struct BatchSolver {
     compose_rowmap(kb, w);
   }
 
+  // The panel's columns kb..kb+w factored by the whole workgroup: thread t
+  // keeps row kb + t (n <= kBatchMaxN = 256 threads).  Per column: wave
+  // argmax (DPP, lowest row among equal maxima; NaN and zero never chosen,
+  // iamax.rs:6-21), the four wave winners reduced in a fixed order through
+  // LDS, then the interchange, reciprocal scaling and rank-1 update with
+  // separate multiply and subtract (getrf.rs:76-88).
+  template <class Mat>
+  __device__ void panel_wg(const Mat &M, int n, int kb, int w, int64_t *ipiv_out) {
+#pragma clang fp contract(off)
+    const int row = kb + t;
+    const bool has = row < n;
+    T v[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) v[c] = (has && c < w) ? M(row, kb + c) : T(0);
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      if (j < w) {
+        const int d = kb + j;
+        T *pr_row = sprow + (j & 1) * 2 * NB;  // [pivot row | diagonal row], by column parity
+        const double x = (has && row >= d) ? fabs((double)v[j]) : 0.0;
+        const int xr = (has && row >= d && x > 0.0) ? row : INT_MAX;
+        const double vmax = bwave_max(xr != INT_MAX ? x : 0.0);
+        const unsigned long long m = __ballot(xr != INT_MAX && x == vmax);
+        int wr = INT_MAX;
+        unsigned long long mm = m;
+        while (mm) {
+          const int l = __ffsll((long long)mm) - 1;
+          mm &= mm - 1;
+          const int rl = __builtin_amdgcn_readlane(xr, l);
+          wr = rl < wr ? rl : wr;
+        }
+        if (lane == 0) {
+          s_wv[(j & 1) * 4 + wave] = m ? vmax : 0.0;
+          s_wr[(j & 1) * 4 + wave] = wr;
+        }
+        __syncthreads();
+        double bv = s_wv[(j & 1) * 4];
+        int pr = s_wr[(j & 1) * 4];
+#pragma unroll
+        for (int q = 1; q < 4; ++q) {
+          const double ov = s_wv[(j & 1) * 4 + q];
+          const int orr = s_wr[(j & 1) * 4 + q];
+          if (ov > bv || (ov == bv && orr < pr)) {
+            bv = ov;
+            pr = orr;
+          }
+        }
+        const bool singular = (pr == INT_MAX);
+        if (singular) pr = d;
+        if (row == pr) {
+#pragma unroll
+          for (int c = 0; c < NB; ++c) pr_row[c] = v[c];
+        }
+        if (row == d) {
+#pragma unroll
+          for (int c = 0; c < NB; ++c) pr_row[NB + c] = v[c];
+        }
+        __syncthreads();
+        if (pr != d) {
+          if (row == d) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) v[c] = pr_row[c];
+          } else if (row == pr) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) v[c] = pr_row[NB + c];
+          }
+        }
+        if (t == 0) {
+          spiv[j] = pr;
+          if (singular) *sinfo = d;
+        }
+        if (!singular && has && row > d) {
+          const T recip = T(1) / pr_row[j];
+          const T l = v[j] * recip;
+          v[j] = l;
+#pragma unroll
+          for (int c = j + 1; c < NB; ++c) {
+            const T e = l * pr_row[c];
+            v[c] = v[c] - e;
+          }
+        }
+      }
+    }
+    if (has) {
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        if (c < w) {
+          M(row, kb + c) = v[c];
+          sL[t * NB + c] = v[c];
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+      for (int e = 0; e < w; ++e) ipiv_out[kb + e] = spiv[e];
+      compose_rowmap(kb, w);
+    }
+  }
+
   template <class Mat>
   __device__ void solve(const Mat &M, int n, int r, int64_t *ipiv_out) {
     const int nc = n + r;
     if (t == 0) *sinfo = -1;
     for (int kb = 0; kb < n; kb += NB) {
       const int w = n - kb < NB ? n - kb : NB;
-      if (wave == 0) panel(M, n, kb, w, ipiv_out);
+      panel_wg(M, n, kb, w, ipiv_out);
       __syncthreads();
       // Interchanges on the columns of [A | B] outside the panel.
       const int cnt = *scnt;
@@ -359,10 +459,12 @@ __global__ void __launch_bounds__(256) batched_gesv_kernel(BatchArgs<T> p) {
   T *sL = reinterpret_cast<T *>(smem_raw);
   T *sU = sL + kBatchMaxN * NB;
   T *sprow = sU + NB * kBatchLDU;
-  T *sM = sprow + 2 * NB;  // augmented matrix when it fits
+  T *sM = sprow + 4 * NB;  // augmented matrix when it fits
   __shared__ int spiv[NB];
   __shared__ int srows[2 * NB], ssrc[2 * NB];
   __shared__ int scnt, sinfo, ssys;
+  __shared__ double s_wv[8];
+  __shared__ int s_wr[8];
   BatchSolver<T, NB> S;
   S.t = threadIdx.x;
   S.lane = threadIdx.x & 63;
@@ -375,6 +477,8 @@ __global__ void __launch_bounds__(256) batched_gesv_kernel(BatchArgs<T> p) {
   S.ssrc = ssrc;
   S.scnt = &scnt;
   S.sinfo = &sinfo;
+  S.s_wv = s_wv;
+  S.s_wr = s_wr;
   for (;;) {
     if (threadIdx.x == 0) ssys = atomicAdd(p.queue, 1);
     __syncthreads();
@@ -410,7 +514,7 @@ __global__ void __launch_bounds__(256) batched_gesv_kernel(BatchArgs<T> p) {
 
 template <typename T, int NB>
 static size_t batched_lds_bytes(int lds_max_elems) {
-  return sizeof(T) * ((size_t)kBatchMaxN * NB + (size_t)NB * kBatchLDU + 2 * NB + lds_max_elems);
+  return sizeof(T) * ((size_t)kBatchMaxN * NB + (size_t)NB * kBatchLDU + 4 * NB + lds_max_elems);
 }
 
 __global__ void zero_int_kernel(int *p) { *p = 0; }
@@ -467,7 +571,7 @@ template hipError_t launch_batched_gesv<float>(lair_context *, const BatchLaunch
 
 size_t batched_lds_bytes_for(int elem_size, int nb, int lds_max_elems) {
   return (size_t)elem_size *
-         ((size_t)kBatchMaxN * nb + (size_t)nb * kBatchLDU + 2 * nb + lds_max_elems);
+         ((size_t)kBatchMaxN * nb + (size_t)nb * kBatchLDU + 4 * nb + lds_max_elems);
 }
 
 }  // namespace lair
