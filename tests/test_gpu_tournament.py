@@ -31,13 +31,13 @@ def ctx():
     c.close()
 
 
-def factor(ctx, a32, prec=_capi.PREC_FP32, nb=0):
+def factor(ctx, a32, prec=_capi.PREC_FP32, nb=0, pivoting=_capi.PIVOT_TOURNAMENT):
     m, n = a32.shape
     da = L.DeviceArray.from_host(a32)
     dp = L.DeviceArray(min(m, n), 1, np.int64)
     sing = ctypes.c_int64(-2)
     L.check(ctx.lib.lair_sgetrf_pivot_device(ctx.h, m, n, da.ptr, da.ld, dp.ptr, nb,
-                                             _capi.PIVOT_TOURNAMENT, prec, ctypes.byref(sing)),
+                                             pivoting, prec, ctypes.byref(sing)),
             "sgetrf_pivot")
     return da.download(), dp.download().ravel(), sing.value
 
@@ -95,3 +95,13 @@ def test_mixed_solver_with_tournament_pivoting(ctx, prec):
     assert st == _capi.LAIR_OK, _capi.last_error()
     assert info.converged == 1 and info.fell_back == 0
     assert host_berr(a, x, b) <= n * np.finfo(np.float64).eps
+
+
+def test_partial_pivoting_fp32_tall_panels_reconstruct(ctx):
+    # m > 4096: the FP32 exact-pivoting leaf over several 2048-row workgroups.
+    m, n = 9000, 160
+    a = inputs.gaussian_matrix(4242, m, n).astype(np.float32)
+    lu, piv, sing = factor(ctx, a, pivoting=0)
+    assert sing == -1
+    lmax = check_plu(a, lu, piv)
+    assert lmax <= 1.0  # partial pivoting: |L| <= 1
