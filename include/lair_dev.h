@@ -114,7 +114,11 @@ lair_status lair_dsolve(int64_t m, int64_t n, const double *a, int64_t rs, int64
 /* ----------------------------------------------- device-resident calls --- */
 
 /* getrf on a device-resident row-major matrix with leading dimension lda.
- * nb = panel width (0: autotuned).  ipiv is a device int64 array of min(m,n)
+ * nb = panel width: > 0 that width (capped at 256), 0 the default (128, or
+ * 64 when min(m,n) <= 256), < 0 autotuned: widths 64, 128 and 256 each factor
+ * a scratch copy once and the fastest is kept per shape in the context (the
+ * first such call costs three extra factorizations).  Widths change only the
+ * summation order of the trailing updates.  ipiv is a device int64 array of min(m,n)
  * entries.  *singular (host) is written after the call completes (the call
  * synchronizes the stream when singular != NULL). */
 lair_status lair_dgetrf_device(lair_context *ctx, int64_t m, int64_t n, double *a, int64_t lda,

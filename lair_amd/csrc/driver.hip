@@ -239,6 +239,65 @@ static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a,
   return LAIR_OK;
 }
 
+// Panel-width autotuner: each candidate width in {64, 128, 256} factors a
+// scratch copy of the matrix once (timed with events on the context's
+// stream); the fastest is kept per (m, n, element size, mode) for the life of
+// the context.
+template <typename T>
+static lair_status tune_nb(lair_context *ctx, int64_t m, int64_t n, const T *a, int64_t lda, int mode,
+                           int64_t *nb_out) {
+  const std::array<int64_t, 4> key{m, n, (int64_t)sizeof(T), (int64_t)mode};
+  auto it = ctx->nb_cache.find(key);
+  if (it != ctx->nb_cache.end()) {
+    *nb_out = it->second;
+    return LAIR_OK;
+  }
+  if (m <= 0 || n <= 0 || lda < n) {
+    *nb_out = 0;
+    return LAIR_OK;
+  }
+  const size_t bytes = sizeof(T) * (size_t)m * lda;
+  void *scratch = nullptr, *sp = nullptr;
+  LAIR_HIP(hipMalloc(&scratch, bytes));
+  if (hipMalloc(&sp, sizeof(int64_t) * (size_t)std::min(m, n)) != hipSuccess) {
+    hipFree(scratch);
+    return hip_status(hipErrorOutOfMemory, "autotune");
+  }
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  int best = 128;
+  float best_ms = 1e30f;
+  lair_status st = LAIR_OK;
+  for (int cand : {64, 128, 256}) {
+    if (cand > std::min(m, n) && cand != 64) continue;
+    hipMemcpyAsync(scratch, a, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+    hipEventRecord(e0, ctx->stream);
+    st = getrf_impl<T>(ctx, m, n, static_cast<T *>(scratch), lda, static_cast<int64_t *>(sp), cand, mode,
+                       nullptr);
+    if (st != LAIR_OK) break;
+    hipEventRecord(e1, ctx->stream);
+    if (hipEventSynchronize(e1) != hipSuccess) {
+      st = hip_status(hipErrorUnknown, "autotune");
+      break;
+    }
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = cand;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(scratch);
+  hipFree(sp);
+  if (st != LAIR_OK) return st;
+  ctx->nb_cache[key] = best;
+  *nb_out = best;
+  return LAIR_OK;
+}
+
 // Solve-side data derived from a factorization: the row permutation and the
 // inverses of the diagonal blocks of L and U.
 template <typename T>
@@ -583,6 +642,10 @@ lair_status lair_dgetrf_device(lair_context *ctx, int64_t m, int64_t n, double *
                                int64_t *ipiv, int64_t nb, int64_t *singular) {
   if (!ctx) return -1;
   std::lock_guard<std::mutex> lk(ctx->mu);
+  if (nb < 0) {
+    lair_status st = tune_nb<double>(ctx, m, n, a, lda, kGemmF64, &nb);
+    if (st != LAIR_OK) return st;
+  }
   return getrf_impl<double>(ctx, m, n, a, lda, ipiv, nb, kGemmF64, singular);
 }
 
@@ -590,6 +653,10 @@ lair_status lair_sgetrf_device(lair_context *ctx, int64_t m, int64_t n, float *a
                                int64_t *ipiv, int64_t nb, int64_t *singular) {
   if (!ctx) return -1;
   std::lock_guard<std::mutex> lk(ctx->mu);
+  if (nb < 0) {
+    lair_status st = tune_nb<float>(ctx, m, n, a, lda, kGemmF32, &nb);
+    if (st != LAIR_OK) return st;
+  }
   return getrf_impl<float>(ctx, m, n, a, lda, ipiv, nb, kGemmF32, singular);
 }
 

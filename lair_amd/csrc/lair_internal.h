@@ -4,7 +4,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -104,6 +106,9 @@ struct lair_context {
   lair::Buffer ws_perm, ws_linv, ws_uinv, ws_flags, ws_tmp2;
   size_t ws_flags_zeroed = 0;
   uint32_t tri_epoch = 0;
+
+  // Panel widths chosen by the getrf autotuner, per (m, n, element size, mode).
+  std::map<std::array<int64_t, 4>, int> nb_cache;
 
   int32_t prof_mask = 0;  // bit k: record kernel class k
   std::vector<lair::ProfRecord> prof;

@@ -176,7 +176,8 @@ def test_c1_fixture_n512():
     assert lu_backward_error(a, g, piv) <= 8 * fx["n"] * EPS
 
 
-def test_c2_fixture_n8192_device():
+@pytest.mark.parametrize("nb", [0, 64, 256, -1])  # default width, fixed widths, autotuned
+def test_c2_fixture_n8192_device(nb):
     path = os.path.join(HERE, "golden", "c2_n8192.json")
     fx = json.load(open(path))
     n, nrhs = fx["n"], fx["nrhs"]
@@ -189,7 +190,7 @@ def test_c2_fixture_n8192_device():
         dp = L.DeviceArray(n, 1, np.int64)
         import ctypes
         sing = ctypes.c_int64(-1)
-        L.check(ctx.lib.lair_dgetrf_device(ctx.h, n, n, da.ptr, ld, dp.ptr, 0, ctypes.byref(sing)), "getrf")
+        L.check(ctx.lib.lair_dgetrf_device(ctx.h, n, n, da.ptr, ld, dp.ptr, nb, ctypes.byref(sing)), "getrf")
         assert sing.value == -1
         piv = dp.download().reshape(-1)
         np.testing.assert_array_equal(piv, np.array(fx["pivots"]))
