@@ -10,12 +10,13 @@
 // factorization; a blocked back substitution with U gives X (getrs.rs:30-36).
 //
 // Per panel of NB columns:
-//  * wave 0 keeps the panel rows in registers (lane l: rows l, l+64, ...) and
-//    does, per column, a DPP argmax (ties -> lowest row; NaN and zero never
-//    chosen, src/blas/iamax.rs:6-21), the interchange, the reciprocal scaling
-//    and the rank-1 update (getrf.rs:76-88);
-//  * the panel's interchanges compose into one row map, applied to the other
-//    columns of [A | B] as a gather;
+//  * thread t keeps panel row kb + t in registers; per column, a DPP argmax
+//    per wave and a fixed-order reduction of the four wave winners (ties ->
+//    lowest row; NaN and zero never chosen, src/blas/iamax.rs:6-21), the
+//    interchange, the reciprocal scaling and the rank-1 update
+//    (getrf.rs:76-88);
+//  * the panel's interchanges form one row map, applied to the other columns
+//    of [A | B] as a gather;
 //  * U12 = L11^-1 A12 column by column, then A22 -= L21 U12 with L21 and U12
 //    staged in LDS and 4 x 4 register tiles.
 // The augmented matrix lives in LDS when it fits (copied in and out), else it
@@ -90,176 +91,35 @@ __device__ __forceinline__ double bwave_max(double v) {
   return __hiloint2double(hi, lo);
 }
 
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 template <typename T, int NB>
 struct BatchSolver {
-  static constexpr int RPT = kBatchMaxN / 64;  // panel rows per lane
   int t, lane, wave;
   T *sL;      // [kBatchMaxN][NB]: L11 and L21 of the current panel
   T *sU;      // [NB][kBatchLDU]: U12 (and the rows of B) of the current panel
   T *sprow;   // [2][2][NB]: pivot row and diagonal row, by column parity
   int *spiv;  // [NB]
   int *srows, *ssrc, *scnt;  // row map of the panel's interchanges
+  int *s_srcx;                // [2][2]: origins of the pivot and diagonal rows, by column parity
   int *sinfo;
   double *s_wv;  // [2][4] wave maxima (workgroup panel), by column parity
   int *s_wr;     // [2][4] their rows
-
-  // Row map of the interchanges spiv[0..w) (absolute rows >= k0): rows[e]
-  // receives the contents of src[e].  Lane 0 of wave 0 composes it.
-  __device__ void compose_rowmap(int k0, int w) {
-    if (lane != 0) return;
-    int cnt = w;
-    for (int e = 0; e < w; ++e) {
-      srows[e] = k0 + e;
-      ssrc[e] = k0 + e;
-    }
-    for (int e = 0; e < w; ++e) {
-      const int p = spiv[e];
-      if (p == k0 + e) continue;
-      int ep = -1;
-      if (p < k0 + w) {
-        ep = p - k0;
-      } else {
-        for (int q = w; q < cnt; ++q)
-          if (srows[q] == p) {
-            ep = q;
-            break;
-          }
-        if (ep < 0) {
-          ep = cnt++;
-          srows[ep] = p;
-          ssrc[ep] = p;
-        }
-      }
-      const int tmp = ssrc[e];
-      ssrc[e] = ssrc[ep];
-      ssrc[ep] = tmp;
-    }
-    *scnt = cnt;
-  }
-
-  template <class Mat>
-  __device__ void panel(const Mat &M, int n, int kb, int w, int64_t *ipiv_out) {
-#pragma clang fp contract(off)
-    T v[RPT][NB];
-#pragma unroll
-    for (int s = 0; s < RPT; ++s) {
-      const int row = kb + lane + 64 * s;
-#pragma unroll
-      for (int c = 0; c < NB; ++c) v[s][c] = (row < n && c < w) ? M(row, kb + c) : T(0);
-    }
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-      if (j < w) {
-        const int d = kb + j;
-        double bv = 0.0;
-        int br = INT_MAX;
-#pragma unroll
-        for (int s = 0; s < RPT; ++s) {
-          const int row = kb + lane + 64 * s;
-          if (row >= d && row < n) {
-            const double x = fabs((double)v[s][j]);
-            if (x > bv) {
-              bv = x;
-              br = row;
-            }
-          }
-        }
-        const double vmax = bwave_max(bv);
-        const unsigned long long m = __ballot(br != INT_MAX && bv == vmax);
-        int pr = INT_MAX;
-        unsigned long long mm = m;
-        while (mm) {
-          const int l = __ffsll((long long)mm) - 1;
-          mm &= mm - 1;
-          const int rl = __builtin_amdgcn_readlane(br, l);
-          pr = rl < pr ? rl : pr;
-        }
-        const bool singular = (pr == INT_MAX);
-        if (singular) pr = d;
-#pragma unroll
-        for (int s = 0; s < RPT; ++s) {
-          const int row = kb + lane + 64 * s;
-          if (row == pr) {
-#pragma unroll
-            for (int c = 0; c < NB; ++c) sprow[c] = v[s][c];
-          }
-          if (row == d) {
-#pragma unroll
-            for (int c = 0; c < NB; ++c) sprow[NB + c] = v[s][c];
-          }
-        }
-        wave_sync();
-        if (pr != d) {
-#pragma unroll
-          for (int s = 0; s < RPT; ++s) {
-            const int row = kb + lane + 64 * s;
-            if (row == d) {
-#pragma unroll
-              for (int c = 0; c < NB; ++c) v[s][c] = sprow[c];
-            } else if (row == pr) {
-#pragma unroll
-              for (int c = 0; c < NB; ++c) v[s][c] = sprow[NB + c];
-            }
-          }
-        }
-        if (lane == 0) {
-          spiv[j] = pr;
-          if (singular) *sinfo = d;
-        }
-        if (!singular) {
-          const T recip = T(1) / sprow[j];
-#pragma unroll
-          for (int s = 0; s < RPT; ++s) {
-            const int row = kb + lane + 64 * s;
-            if (row > d && row < n) {
-              const T l = v[s][j] * recip;
-              v[s][j] = l;
-#pragma unroll
-              for (int c = j + 1; c < NB; ++c) {
-                const T e = l * sprow[c];
-                v[s][c] = v[s][c] - e;
-              }
-            }
-          }
-        }
-        wave_sync();
-      }
-    }
-#pragma unroll
-    for (int s = 0; s < RPT; ++s) {
-      const int row = kb + lane + 64 * s;
-      if (row < n) {
-#pragma unroll
-        for (int c = 0; c < NB; ++c)
-          if (c < w) {
-            M(row, kb + c) = v[s][c];
-            sL[(row - kb) * NB + c] = v[s][c];
-          }
-      }
-    }
-    wave_sync();
-    if (lane == 0)
-      for (int e = 0; e < w; ++e) ipiv_out[kb + e] = spiv[e];
-    compose_rowmap(kb, w);
-  }
 
   // The panel's columns kb..kb+w factored by the whole workgroup: thread t
   // keeps row kb + t (n <= kBatchMaxN = 256 threads).  Per column: wave
   // argmax (DPP, lowest row among equal maxima; NaN and zero never chosen,
   // iamax.rs:6-21), the four wave winners reduced in a fixed order through
   // LDS, then the interchange, reciprocal scaling and rank-1 update with
-  // separate multiply and subtract (getrf.rs:76-88).
+  // separate multiply and subtract (getrf.rs:76-88).  Every thread also
+  // tracks which original row its contents came from; the rows whose origin
+  // changed form the panel's row map (rows[e] receives row src[e]), listed in
+  // any order since it is a permutation of distinct rows.
   template <class Mat>
-  __device__ void panel_wg(const Mat &M, int n, int kb, int w, int64_t *ipiv_out) {
+  __device__ __forceinline__ void panel_wg(const Mat &M, int n, int kb, int w, int64_t *ipiv_out) {
 #pragma clang fp contract(off)
     const int row = kb + t;
     const bool has = row < n;
+    int src = row;
+    if (t == 0) *scnt = 0;
     T v[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c) v[c] = (has && c < w) ? M(row, kb + c) : T(0);
@@ -298,22 +158,27 @@ struct BatchSolver {
         }
         const bool singular = (pr == INT_MAX);
         if (singular) pr = d;
+        int *src_x = s_srcx + (j & 1) * 2;
         if (row == pr) {
 #pragma unroll
           for (int c = 0; c < NB; ++c) pr_row[c] = v[c];
+          src_x[0] = src;
         }
         if (row == d) {
 #pragma unroll
           for (int c = 0; c < NB; ++c) pr_row[NB + c] = v[c];
+          src_x[1] = src;
         }
         __syncthreads();
         if (pr != d) {
           if (row == d) {
 #pragma unroll
             for (int c = 0; c < NB; ++c) v[c] = pr_row[c];
+            src = src_x[0];
           } else if (row == pr) {
 #pragma unroll
             for (int c = 0; c < NB; ++c) v[c] = pr_row[NB + c];
+            src = src_x[1];
           }
         }
         if (t == 0) {
@@ -339,16 +204,18 @@ struct BatchSolver {
           M(row, kb + c) = v[c];
           sL[t * NB + c] = v[c];
         }
+      if (src != row) {
+        const int e = atomicAdd(scnt, 1);
+        srows[e] = row;
+        ssrc[e] = src;
+      }
     }
     __syncthreads();
-    if (t == 0) {
-      for (int e = 0; e < w; ++e) ipiv_out[kb + e] = spiv[e];
-      compose_rowmap(kb, w);
-    }
+    if (t < w) ipiv_out[kb + t] = spiv[t];
   }
 
   template <class Mat>
-  __device__ void solve(const Mat &M, int n, int r, int64_t *ipiv_out) {
+  __device__ __forceinline__ void solve(const Mat &M, int n, int r, int64_t *ipiv_out) {
     const int nc = n + r;
     if (t == 0) *sinfo = -1;
     for (int kb = 0; kb < n; kb += NB) {
@@ -395,11 +262,16 @@ struct BatchSolver {
         const int tr = (rows + 3) / 4, tc = (cols + 3) / 4;
         for (int tile = t; tile < tr * tc; tile += 256) {
           const int i0 = (tile / tc) * 4, j0 = (tile % tc) * 4;
-          T acc[4][4];
+          // The tile of A22 is loaded first so that its latency overlaps the
+          // products from LDS.
+          T cv[4][4], acc[4][4];
 #pragma unroll
           for (int a = 0; a < 4; ++a)
 #pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
+            for (int b = 0; b < 4; ++b) {
+              cv[a][b] = (i0 + a < rows && j0 + b < cols) ? M(c0 + i0 + a, c0 + j0 + b) : T(0);
+              acc[a][b] = T(0);
+            }
 #pragma unroll
           for (int q = 0; q < NB; ++q) {
             if (q < w) {
@@ -419,34 +291,63 @@ struct BatchSolver {
             if (i0 + a < rows) {
 #pragma unroll
               for (int b = 0; b < 4; ++b)
-                if (j0 + b < cols) {
-                  T &cv = M(c0 + i0 + a, c0 + j0 + b);
-                  cv = cv - acc[a][b];
-                }
+                if (j0 + b < cols) M(c0 + i0 + a, c0 + j0 + b) = cv[a][b] - acc[a][b];
             }
         }
       }
       __syncthreads();
     }
     // Back substitution U X = Y on the right-hand-side columns, blocked from
-    // the bottom: diagonal block (true division), then the rows above.
+    // the bottom (getrs.rs:30-36, true division).  Per block of NB rows: the
+    // diagonal block of U and the block's rows of Y are staged in LDS, one
+    // thread per right-hand side solves them in registers, then one thread
+    // per row above subtracts the block's contribution.
+    T *sD = sL;                // [NB][NB]: diagonal block of U
+    T *sX = sL + NB * NB;      // [NB][kBatchMaxRhs]: the block's rows of X
     for (int kb = ((n - 1) / NB) * NB; kb >= 0; kb -= NB) {
       const int w = n - kb < NB ? n - kb : NB;
-      if (t < r) {
-        const int c = n + t;
-        for (int i = w - 1; i >= 0; --i) {
-          const int gi = kb + i;
-          T acc = M(gi, c);
-          for (int q = i + 1; q < w; ++q) acc -= M(gi, kb + q) * M(kb + q, c);
-          M(gi, c) = acc / M(gi, gi);
-        }
+      for (int e = t; e < w * w; e += 256) {
+        const int i = e / w, q = e % w;
+        if (q >= i) sD[i * NB + q] = M(kb + i, kb + q);
+      }
+      for (int e = t; e < w * r; e += 256) {
+        const int i = e / r, c = e % r;
+        sX[i * kBatchMaxRhs + c] = M(kb + i, n + c);
       }
       __syncthreads();
-      for (int e = t; e < kb * r; e += 256) {
-        const int i = e / r, c = n + e % r;
-        T acc = M(i, c);
-        for (int q = 0; q < w; ++q) acc -= M(i, kb + q) * M(kb + q, c);
-        M(i, c) = acc;
+      if (t < r) {
+        T x[NB];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) x[i] = i < w ? sX[i * kBatchMaxRhs + t] : T(0);
+#pragma unroll
+        for (int i = NB - 1; i >= 0; --i) {
+          if (i < w) {
+            T acc = x[i];
+#pragma unroll
+            for (int q = i + 1; q < NB; ++q)
+              if (q < w) acc -= sD[i * NB + q] * x[q];
+            x[i] = acc / sD[i * NB + i];
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < NB; ++i)
+          if (i < w) {
+            sX[i * kBatchMaxRhs + t] = x[i];
+            M(kb + i, n + t) = x[i];
+          }
+      }
+      __syncthreads();
+      for (int i = t; i < kb; i += 256) {
+        T u[NB];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) u[q] = q < w ? M(i, kb + q) : T(0);
+        for (int c = 0; c < r; ++c) {
+          T acc = M(i, n + c);
+#pragma unroll
+          for (int q = 0; q < NB; ++q)
+            if (q < w) acc -= u[q] * sX[q * kBatchMaxRhs + c];
+          M(i, n + c) = acc;
+        }
       }
       __syncthreads();
     }
@@ -463,6 +364,7 @@ __global__ void __launch_bounds__(256) batched_gesv_kernel(BatchArgs<T> p) {
   __shared__ int spiv[NB];
   __shared__ int srows[2 * NB], ssrc[2 * NB];
   __shared__ int scnt, sinfo, ssys;
+  __shared__ int s_srcx[4];
   __shared__ double s_wv[8];
   __shared__ int s_wr[8];
   BatchSolver<T, NB> S;
@@ -476,6 +378,7 @@ __global__ void __launch_bounds__(256) batched_gesv_kernel(BatchArgs<T> p) {
   S.srows = srows;
   S.ssrc = ssrc;
   S.scnt = &scnt;
+  S.s_srcx = s_srcx;
   S.sinfo = &sinfo;
   S.s_wv = s_wv;
   S.s_wr = s_wr;

@@ -990,7 +990,7 @@ static std::vector<lair::BatchLaunch> batched_candidates(const lair::BatchLaunch
   std::vector<lair::BatchLaunch> out;
   const size_t lds_cap = 160 * 1024;
   for (int nb : {16, 8})
-    for (int per_cu : {1, 2})
+    for (int per_cu : {1, 2, 4})
       for (int frac : {0, 1, 2}) {
         lair::BatchLaunch c = base;
         c.nb = nb;
