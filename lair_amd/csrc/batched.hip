@@ -96,23 +96,28 @@ struct BatchSolver {
   int t, lane, wave;
   T *sL;      // [kBatchMaxN][NB]: L11 and L21 of the current panel
   T *sU;      // [NB][kBatchLDU]: U12 (and the rows of B) of the current panel
-  T *sprow;   // [2][2][NB]: pivot row and diagonal row, by column parity
+  T *scand;   // [2][4][NB]: each wave's candidate pivot row, by column parity
+  T *sdiag;   // [2][NB]: the diagonal row, by column parity
   int *spiv;  // [NB]
   int *srows, *ssrc, *scnt;  // row map of the panel's interchanges
-  int *s_srcx;                // [2][2]: origins of the pivot and diagonal rows, by column parity
+  int *s_srcx;                // [2][5]: origins of the candidate rows and the diagonal row, by column parity
   int *sinfo;
   double *s_wv;  // [2][4] wave maxima (workgroup panel), by column parity
   int *s_wr;     // [2][4] their rows
 
   // The panel's columns kb..kb+w factored by the whole workgroup: thread t
-  // keeps row kb + t (n <= kBatchMaxN = 256 threads).  Per column: wave
-  // argmax (DPP, lowest row among equal maxima; NaN and zero never chosen,
-  // iamax.rs:6-21), the four wave winners reduced in a fixed order through
-  // LDS, then the interchange, reciprocal scaling and rank-1 update with
-  // separate multiply and subtract (getrf.rs:76-88).  Every thread also
-  // tracks which original row its contents came from; the rows whose origin
-  // changed form the panel's row map (rows[e] receives row src[e]), listed in
-  // any order since it is a permutation of distinct rows.
+  // keeps row kb + t (n <= kBatchMaxN = 256 threads).  Per column, one
+  // barrier: every wave finds its argmax with DPP (lowest row among equal
+  // maxima; NaN and zero never chosen, iamax.rs:6-21) and its winning lane
+  // publishes the whole row, while the thread holding the diagonal row
+  // publishes that row; after the barrier every thread reduces the four wave
+  // winners in a fixed order, and the interchange, reciprocal scaling and
+  // rank-1 update (separate multiply and subtract, getrf.rs:76-88) follow
+  // from LDS.  The buffers alternate by column parity, so the next column's
+  // writes cannot overtake this column's reads.  Every thread also tracks
+  // which original row its contents came from; the rows whose origin changed
+  // form the panel's row map (rows[e] receives row src[e]), listed in any
+  // order since it is a permutation of distinct rows.
   template <class Mat>
   __device__ __forceinline__ void panel_wg(const Mat &M, int n, int kb, int w, int64_t *ipiv_out) {
 #pragma clang fp contract(off)
@@ -127,7 +132,10 @@ struct BatchSolver {
     for (int j = 0; j < NB; ++j) {
       if (j < w) {
         const int d = kb + j;
-        T *pr_row = sprow + (j & 1) * 2 * NB;  // [pivot row | diagonal row], by column parity
+        const int par = j & 1;
+        T *cand = scand + par * 4 * NB;
+        T *diag = sdiag + par * NB;
+        int *srcs = s_srcx + par * 5;  // [4 candidates | diagonal]
         const double x = (has && row >= d) ? fabs((double)v[j]) : 0.0;
         const int xr = (has && row >= d && x > 0.0) ? row : INT_MAX;
         const double vmax = bwave_max(xr != INT_MAX ? x : 0.0);
@@ -141,44 +149,44 @@ struct BatchSolver {
           wr = rl < wr ? rl : wr;
         }
         if (lane == 0) {
-          s_wv[(j & 1) * 4 + wave] = m ? vmax : 0.0;
-          s_wr[(j & 1) * 4 + wave] = wr;
+          s_wv[par * 4 + wave] = m ? vmax : 0.0;
+          s_wr[par * 4 + wave] = wr;
+        }
+        if (row == wr) {
+#pragma unroll
+          for (int c = 0; c < NB; ++c) cand[wave * NB + c] = v[c];
+          srcs[wave] = src;
+        }
+        if (row == d) {
+#pragma unroll
+          for (int c = 0; c < NB; ++c) diag[c] = v[c];
+          srcs[4] = src;
         }
         __syncthreads();
-        double bv = s_wv[(j & 1) * 4];
-        int pr = s_wr[(j & 1) * 4];
+        double bv = s_wv[par * 4];
+        int pr = s_wr[par * 4], pw = 0;
 #pragma unroll
         for (int q = 1; q < 4; ++q) {
-          const double ov = s_wv[(j & 1) * 4 + q];
-          const int orr = s_wr[(j & 1) * 4 + q];
+          const double ov = s_wv[par * 4 + q];
+          const int orr = s_wr[par * 4 + q];
           if (ov > bv || (ov == bv && orr < pr)) {
             bv = ov;
             pr = orr;
+            pw = q;
           }
         }
         const bool singular = (pr == INT_MAX);
         if (singular) pr = d;
-        int *src_x = s_srcx + (j & 1) * 2;
-        if (row == pr) {
-#pragma unroll
-          for (int c = 0; c < NB; ++c) pr_row[c] = v[c];
-          src_x[0] = src;
-        }
-        if (row == d) {
-#pragma unroll
-          for (int c = 0; c < NB; ++c) pr_row[NB + c] = v[c];
-          src_x[1] = src;
-        }
-        __syncthreads();
+        const T *prow = cand + pw * NB;  // the pivot row (valid unless singular)
         if (pr != d) {
           if (row == d) {
 #pragma unroll
-            for (int c = 0; c < NB; ++c) v[c] = pr_row[c];
-            src = src_x[0];
+            for (int c = 0; c < NB; ++c) v[c] = prow[c];
+            src = srcs[pw];
           } else if (row == pr) {
 #pragma unroll
-            for (int c = 0; c < NB; ++c) v[c] = pr_row[NB + c];
-            src = src_x[1];
+            for (int c = 0; c < NB; ++c) v[c] = diag[c];
+            src = srcs[4];
           }
         }
         if (t == 0) {
@@ -186,12 +194,12 @@ struct BatchSolver {
           if (singular) *sinfo = d;
         }
         if (!singular && has && row > d) {
-          const T recip = T(1) / pr_row[j];
+          const T recip = T(1) / prow[j];
           const T l = v[j] * recip;
           v[j] = l;
 #pragma unroll
           for (int c = j + 1; c < NB; ++c) {
-            const T e = l * pr_row[c];
+            const T e = l * prow[c];
             v[c] = v[c] - e;
           }
         }
@@ -225,6 +233,7 @@ struct BatchSolver {
       // Interchanges on the columns of [A | B] outside the panel.
       const int cnt = *scnt;
       for (int ci = t; ci < nc - w; ci += 256) {
+        __asm__ volatile("" ::: "memory");  // the row map is re-read per column, not held in registers
         const int c = ci < kb ? ci : ci + w;
         T vals[2 * NB];
 #pragma unroll
@@ -243,6 +252,10 @@ struct BatchSolver {
         for (int i = 0; i < NB; ++i) x[i] = i < w ? M(kb + i, c) : T(0);
 #pragma unroll
         for (int i = 1; i < NB; ++i) {
+          // Row i's loads of L11 are issued after row i - 1 is solved (an
+          // empty asm that uses x[i - 1] and orders memory): scheduled
+          // freely, all NB (NB - 1) / 2 entries would be held in registers.
+          __asm__ volatile("" : "+v"(x[i - 1]) : : "memory");
           T acc = x[i];
 #pragma unroll
           for (int q = 0; q < i; ++q) acc -= sL[i * NB + q] * x[q];
@@ -359,12 +372,13 @@ __global__ void __launch_bounds__(256) batched_gesv_kernel(BatchArgs<T> p) {
   extern __shared__ unsigned char smem_raw[];
   T *sL = reinterpret_cast<T *>(smem_raw);
   T *sU = sL + kBatchMaxN * NB;
-  T *sprow = sU + NB * kBatchLDU;
-  T *sM = sprow + 4 * NB;  // augmented matrix when it fits
+  T *scand = sU + NB * kBatchLDU;
+  T *sdiag = scand + 8 * NB;
+  T *sM = sdiag + 2 * NB;  // augmented matrix when it fits
   __shared__ int spiv[NB];
   __shared__ int srows[2 * NB], ssrc[2 * NB];
   __shared__ int scnt, sinfo, ssys;
-  __shared__ int s_srcx[4];
+  __shared__ int s_srcx[10];
   __shared__ double s_wv[8];
   __shared__ int s_wr[8];
   BatchSolver<T, NB> S;
@@ -373,7 +387,8 @@ __global__ void __launch_bounds__(256) batched_gesv_kernel(BatchArgs<T> p) {
   S.wave = threadIdx.x >> 6;
   S.sL = sL;
   S.sU = sU;
-  S.sprow = sprow;
+  S.scand = scand;
+  S.sdiag = sdiag;
   S.spiv = spiv;
   S.srows = srows;
   S.ssrc = ssrc;
@@ -417,7 +432,7 @@ __global__ void __launch_bounds__(256) batched_gesv_kernel(BatchArgs<T> p) {
 
 template <typename T, int NB>
 static size_t batched_lds_bytes(int lds_max_elems) {
-  return sizeof(T) * ((size_t)kBatchMaxN * NB + (size_t)NB * kBatchLDU + 4 * NB + lds_max_elems);
+  return sizeof(T) * ((size_t)kBatchMaxN * NB + (size_t)NB * kBatchLDU + 10 * NB + lds_max_elems);
 }
 
 __global__ void zero_int_kernel(int *p) { *p = 0; }
@@ -474,7 +489,7 @@ template hipError_t launch_batched_gesv<float>(lair_context *, const BatchLaunch
 
 size_t batched_lds_bytes_for(int elem_size, int nb, int lds_max_elems) {
   return (size_t)elem_size *
-         ((size_t)kBatchMaxN * nb + (size_t)nb * kBatchLDU + 4 * nb + lds_max_elems);
+         ((size_t)kBatchMaxN * nb + (size_t)nb * kBatchLDU + 10 * nb + lds_max_elems);
 }
 
 }  // namespace lair
