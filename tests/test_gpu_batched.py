@@ -20,14 +20,15 @@ def make_batch(count, seed, dtype, nmin=1, nmax=256):
     return ns, rs, As, Bs
 
 
-def run_batch(ctx, ns, rs, As, Bs, dtype, replays=1):
+def run_batch(ctx, ns, rs, As, Bs, dtype, replays=1, autotune=True):
     a = np.concatenate([A.ravel() for A in As]).reshape(-1, 1)
     b = np.concatenate([B.ravel() for B in Bs]).reshape(-1, 1)
     da = L.DeviceArray.from_host(a)
     db = L.DeviceArray.from_host(b)
     dp = L.DeviceArray(int(ns.sum()), 1, np.int64)
     di = L.DeviceArray(len(ns), 1, np.int32)
-    plan = L.BatchedPlan(ctx, ns, rs, da, db, dp, di, precision=0 if dtype == np.float64 else 1)
+    plan = L.BatchedPlan(ctx, ns, rs, da, db, dp, di, precision=0 if dtype == np.float64 else 1,
+                         autotune=autotune)
     outs = []
     for _ in range(replays):
         da.upload(a)
@@ -59,10 +60,16 @@ def ctx():
     c.close()
 
 
+# Without the autotuner the plan keeps its first candidate: 16-column panels
+# with every system updated in global memory; the autotuned plans of these
+# batches stage the smaller systems in LDS.
+@pytest.mark.parametrize("autotune", [True, False])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_batched_matches_oracle(ctx, dtype):
+def test_batched_matches_oracle(ctx, dtype, autotune):
     ns, rs, As, Bs = make_batch(300, 11 if dtype == np.float64 else 12, dtype)
-    outs, cfg = run_batch(ctx, ns, rs, As, Bs, dtype)
+    outs, cfg = run_batch(ctx, ns, rs, As, Bs, dtype, autotune=autotune)
+    if not autotune:
+        assert cfg["nb"] == 16 and cfg["lds_elems"] == 0
     lu, x, piv, info = outs[0]
     eps = np.finfo(dtype).eps
     for i, (lui, xi, pi) in enumerate(split(ns, rs, lu, x, piv)):
