@@ -241,6 +241,9 @@ def main():
                     help="A ~ N(0,1) from the library's own generator kernel instead of the kappa=1e6 "
                          "construction (same factorization shapes; for counter-collection runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="only the batched (C4) measurements, FP64 then FP32, one JSON line each "
+                         "(profiling runs of the batched kernel)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the n=16384 LU "
@@ -249,6 +252,14 @@ def main():
 
     import lair_amd as L
     from lair_amd import _capi
+
+    if args.only_batched:
+        ctx = L.Context()
+        gl = gen_lib()
+        for dt in (np.float64, np.float32):
+            print(json.dumps({"batched_c4": np.dtype(dt).name, **bench_batched(ctx, gl, dt, args.steps)}),
+                  flush=True)
+        return
 
     n, nrhs, K, W = args.n, args.nrhs, args.steps, args.warmup
     ld = n + 64 if (n % 4096 == 0) else ((n + 63) // 64) * 64
