@@ -14,12 +14,6 @@
 
 #include "lair_internal.h"
 
-// Phase marker inside the panel kernel: expands to nothing in the library;
-// a profiling build may define it to record a timestamp.
-#ifndef LAIR_PANEL_MARK
-#define LAIR_PANEL_MARK(col, phase)
-#endif
-
 namespace lair {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -29,12 +23,6 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 // Bounded wait: a cooperative panel gives up (and flags the error) after this
 // many polls instead of hanging when a peer never arrives.
 constexpr int kSpinLimit = 1 << 23;
-
-#ifndef LAIR_CALU_MARK
-#define LAIR_CALU_MARK(phase) \
-  do {                        \
-  } while (0)
-#endif
 
 template <typename T>
 __device__ __forceinline__ void st_agent(T *p, T v) {
@@ -313,7 +301,6 @@ struct PanelLeaf {
     double wv;
     int wr;
     wave_argmax(bv, br, wv, wr);
-    LAIR_PANEL_MARK(J, 1);
 #pragma unroll
     for (int s = 0; s < RPT; ++s) {
       const int r = base + t + s * NT;
@@ -331,7 +318,6 @@ struct PanelLeaf {
       s_wr[par][wave] = wr;
     }
     __syncthreads();
-    LAIR_PANEL_MARK(J, 2);
     bv = s_wv[par][0];
     br = s_wr[par][0];
     int qw = 0;
@@ -355,7 +341,6 @@ struct PanelLeaf {
       win = s_win[par];
       prow = s_prow[par];
     }
-    LAIR_PANEL_MARK(J, 3);
     const bool singular = (win == INT_MAX);
     const int pr = singular ? d : win;
     if (pr != d) {
@@ -392,7 +377,6 @@ struct PanelLeaf {
         }
       }
     }
-    LAIR_PANEL_MARK(J, 4);
     // No trailing barrier: the next column uses the other parity's buffers,
     // and the one after that is written only after the next column's first
     // barrier, which every thread reaches after finishing this update.
@@ -445,9 +429,7 @@ __global__ void __launch_bounds__(NT) panel_coop_kernel(PanelArgs<T> pa) {
 
   PanelLeaf<T, W, RPT, NT> leaf{pa, v, s_wv, s_wr, s_wrow, s_prow, s_drow, s_win, s_piv, s_rec, &s_bail,
                                 g, t, t & 63, t >> 6, base};
-  LAIR_PANEL_MARK(0, 0);
   if (!ColumnLoop<0, W>::run(leaf)) return;
-  LAIR_PANEL_MARK(W, 0);
 
 #pragma unroll
   for (int s = 0; s < RPT; ++s) {
@@ -1438,7 +1420,6 @@ __global__ void __launch_bounds__(512) calu_leaf_kernel(CaluArgs ca) {
   __shared__ int s_bail;
 
   const bool vec = rows_aligned(ca.a, ca.lda, ca.col0);
-  LAIR_CALU_MARK(0);
   // 1. Local elimination on a register copy of this thread's rows.
   {
     float v[K][W];
@@ -1454,7 +1435,6 @@ __global__ void __launch_bounds__(512) calu_leaf_kernel(CaluArgs ca) {
   if (lane == 0)
     for (int j = npiv; j < W; ++j) s_nid[wave][j] = INT_MAX;
   __syncthreads();
-  LAIR_CALU_MARK(1);
   // Nominees' original values (still unmodified in memory).
   for (int i = lane; i < W * W; i += 64) {
     const int e = i / W, w = i % W;
@@ -1470,7 +1450,6 @@ __global__ void __launch_bounds__(512) calu_leaf_kernel(CaluArgs ca) {
     }
   }
   __syncthreads();
-  LAIR_CALU_MARK(2);
   // 2. Workgroup round: 8 x 16 nominees, 2 per lane.
   if (wave == 0) {
     float v[2][W];
@@ -1491,7 +1470,6 @@ __global__ void __launch_bounds__(512) calu_leaf_kernel(CaluArgs ca) {
     }
   }
   __syncthreads();
-  LAIR_CALU_MARK(3);
   if (ca.P > 1) {
     // 3. Publish the workgroup's nominees (and, from workgroup 0, the
     // original diagonal rows); gather everyone's.
@@ -1523,7 +1501,6 @@ __global__ void __launch_bounds__(512) calu_leaf_kernel(CaluArgs ca) {
     }
     __syncthreads();
     if (s_bail) return;
-    LAIR_CALU_MARK(4);
     // Final round on the P x 16 finalists.
     if (wave == 0) {
       const int nf = ca.P * W;
@@ -1555,7 +1532,6 @@ __global__ void __launch_bounds__(512) calu_leaf_kernel(CaluArgs ca) {
     }
     __syncthreads();
   }
-  LAIR_CALU_MARK(5);
   // 4. Pivot vector and row map: position col0 + i receives row s_fid[i]
   // (sequential interchanges, LAPACK convention).
   if (t == 0) {
@@ -1642,7 +1618,6 @@ __global__ void __launch_bounds__(512) calu_leaf_kernel(CaluArgs ca) {
     }
     store_row<float, W>(ca.a + (int64_t)r * ca.lda + ca.col0, x, ca.wcols, vec);
   }
-  LAIR_CALU_MARK(6);
   if (g != 0 || (ca.rest1 <= ca.col0 + ca.wcols && ca.left0 >= ca.col0)) return;
   // 6. Rest of the panel (workgroup 0; see panel_coop_kernel): interchanges
   // on [left0, col0) and [col0 + wcols, rest1), U12 = L11^-1 A12 on the latter.
