@@ -367,8 +367,12 @@ struct BatchSolver {
   }
 };
 
-template <typename T, int NB>
-__global__ void __launch_bounds__(256) batched_gesv_kernel(BatchArgs<T> p) {
+// OCC > 1 caps the registers so that OCC workgroups fit on a CU (each wave
+// of a 256-thread workgroup runs on its own SIMD); values beyond the cap
+// spill to scratch.  More resident workgroups hide more memory latency.
+template <typename T, int NB, int OCC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
+batched_gesv_kernel(BatchArgs<T> p) {
   extern __shared__ unsigned char smem_raw[];
   T *sL = reinterpret_cast<T *>(smem_raw);
   T *sU = sL + kBatchMaxN * NB;
@@ -458,12 +462,23 @@ hipError_t launch_batched_gesv(lair_context *ctx, const BatchLaunch &bl, T *a, T
   // be captured in a graph and replayed.
   zero_int_kernel<<<1, 1, 0, ctx->stream>>>(bl.queue);
   const unsigned grid = (unsigned)bl.grid;
+  const size_t shm = bl.nb == 8 ? batched_lds_bytes<T, 8>(bl.lds_max_elems)
+                                : batched_lds_bytes<T, 16>(bl.lds_max_elems);
+  auto go = [&](auto kernel) { kernel<<<grid, 256, shm, ctx->stream>>>(p); };
   if (bl.nb == 8) {
-    const size_t shm = batched_lds_bytes<T, 8>(bl.lds_max_elems);
-    batched_gesv_kernel<T, 8><<<grid, 256, shm, ctx->stream>>>(p);
+    if (bl.occ == 4)
+      go(batched_gesv_kernel<T, 8, 4>);
+    else if (bl.occ == 3)
+      go(batched_gesv_kernel<T, 8, 3>);
+    else
+      go(batched_gesv_kernel<T, 8, 1>);
   } else {
-    const size_t shm = batched_lds_bytes<T, 16>(bl.lds_max_elems);
-    batched_gesv_kernel<T, 16><<<grid, 256, shm, ctx->stream>>>(p);
+    if (bl.occ == 4)
+      go(batched_gesv_kernel<T, 16, 4>);
+    else if (bl.occ == 3)
+      go(batched_gesv_kernel<T, 16, 3>);
+    else
+      go(batched_gesv_kernel<T, 16, 1>);
   }
   return hipGetLastError();
 }
@@ -473,10 +488,18 @@ hipError_t launch_batched_gesv(lair_context *ctx, const BatchLaunch &bl, T *a, T
 void batched_prepare() {
   static bool done = false;
   if (done) return;
-  for (const void *f : {reinterpret_cast<const void *>(batched_gesv_kernel<double, 8>),
-                        reinterpret_cast<const void *>(batched_gesv_kernel<double, 16>),
-                        reinterpret_cast<const void *>(batched_gesv_kernel<float, 8>),
-                        reinterpret_cast<const void *>(batched_gesv_kernel<float, 16>)})
+  for (const void *f : {reinterpret_cast<const void *>(batched_gesv_kernel<double, 8, 1>),
+                        reinterpret_cast<const void *>(batched_gesv_kernel<double, 8, 3>),
+                        reinterpret_cast<const void *>(batched_gesv_kernel<double, 8, 4>),
+                        reinterpret_cast<const void *>(batched_gesv_kernel<double, 16, 1>),
+                        reinterpret_cast<const void *>(batched_gesv_kernel<double, 16, 3>),
+                        reinterpret_cast<const void *>(batched_gesv_kernel<double, 16, 4>),
+                        reinterpret_cast<const void *>(batched_gesv_kernel<float, 8, 1>),
+                        reinterpret_cast<const void *>(batched_gesv_kernel<float, 8, 3>),
+                        reinterpret_cast<const void *>(batched_gesv_kernel<float, 8, 4>),
+                        reinterpret_cast<const void *>(batched_gesv_kernel<float, 16, 1>),
+                        reinterpret_cast<const void *>(batched_gesv_kernel<float, 16, 3>),
+                        reinterpret_cast<const void *>(batched_gesv_kernel<float, 16, 4>)})
     (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
   (void)hipGetLastError();
   done = true;

@@ -984,16 +984,18 @@ static hipError_t batched_launch(lair_batched_plan *pl, const lair::BatchLaunch 
 
 // Candidate launch shapes: panel width, LDS budget for whole-system-in-LDS,
 // workgroups per CU.  The LDS budget trades systems solved on chip against
-// workgroups resident per CU.
+// workgroups resident per CU; 3 and 4 workgroups per CU use kernels whose
+// registers are capped to fit that many.
 static std::vector<lair::BatchLaunch> batched_candidates(const lair::BatchLaunch &base, int esize,
                                                         int ncu) {
   std::vector<lair::BatchLaunch> out;
   const size_t lds_cap = 160 * 1024;
   for (int nb : {16, 8})
-    for (int per_cu : {1, 2, 4})
+    for (int per_cu : {1, 2, 3, 4})
       for (int frac : {0, 1, 2}) {
         lair::BatchLaunch c = base;
         c.nb = nb;
+        c.occ = per_cu >= 3 ? per_cu : 1;
         const size_t fixed = batched_lds_bytes_for(esize, nb, 0);
         const size_t per_wg = lds_cap / per_cu;
         if (per_wg <= fixed + 1024) continue;

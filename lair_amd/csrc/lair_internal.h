@@ -220,6 +220,7 @@ struct BatchLaunch {
   int lds_max_elems = 0;  // largest augmented matrix solved in LDS
   int nb = 16;            // panel width (8 or 16)
   int grid = 256;         // persistent workgroups
+  int occ = 1;            // register budget for this many waves per SIMD (1: the compiler's choice)
 };
 template <typename T>
 hipError_t launch_batched_gesv(lair_context *ctx, const BatchLaunch &bl, T *a, T *b, int64_t *ipiv,
