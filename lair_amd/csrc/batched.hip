@@ -311,27 +311,45 @@ struct BatchSolver {
       __syncthreads();
     }
     // Back substitution U X = Y on the right-hand-side columns, blocked from
-    // the bottom (getrs.rs:30-36, true division).  Per block of NB rows: the
-    // diagonal block of U and the block's rows of Y are staged in LDS, one
-    // thread per right-hand side solves them in registers, then one thread
-    // per row above subtracts the block's contribution.
-    T *sD = sL;                // [NB][NB]: diagonal block of U
-    T *sX = sL + NB * NB;      // [NB][kBatchMaxRhs]: the block's rows of X
-    for (int kb = ((n - 1) / NB) * NB; kb >= 0; kb -= NB) {
+    // the bottom (getrs.rs:30-36, true division).  Y stays in LDS from start
+    // to end.  Per block of NB rows, one thread per right-hand side solves the
+    // diagonal block in registers, then one thread per row above subtracts
+    // the block's contribution.  The next block's diagonal entries and row
+    // segments of U are loaded into registers while the current block is
+    // solved, so the chain of blocks carries no memory round trip.
+    static_assert(kBatchMaxN * kBatchMaxRhs + NB * NB <= kBatchMaxN * NB + NB * kBatchLDU,
+                  "Y and the diagonal block fit the panel buffers");
+    T *sY = sL;                               // [n][r]: Y, then X
+    T *sD = sL + kBatchMaxN * kBatchMaxRhs;   // [NB][NB]: diagonal block of U
+    for (int e = t; e < n * r; e += 256) sY[e] = M(e / r, n + e % r);
+    auto load_d = [&](int kb) -> T {
+      const int w = n - kb < NB ? n - kb : NB, i = t / NB, q = t % NB;
+      return (t < NB * NB && i < w && q < w && q >= i) ? M(kb + i, kb + q) : T(0);
+    };
+    auto load_u = [&](int kb, T (&u)[NB]) {
       const int w = n - kb < NB ? n - kb : NB;
-      for (int e = t; e < w * w; e += 256) {
-        const int i = e / w, q = e % w;
-        if (q >= i) sD[i * NB + q] = M(kb + i, kb + q);
-      }
-      for (int e = t; e < w * r; e += 256) {
-        const int i = e / r, c = e % r;
-        sX[i * kBatchMaxRhs + c] = M(kb + i, n + c);
+#pragma unroll
+      for (int q = 0; q < NB; ++q) u[q] = (t < kb && q < w) ? M(t, kb + q) : T(0);
+    };
+    int kb = ((n - 1) / NB) * NB;
+    T dcur = load_d(kb), ucur[NB];
+    load_u(kb, ucur);
+    for (;;) {
+      const int w = n - kb < NB ? n - kb : NB;
+      const int kn = kb - NB;
+      if (t < NB * NB) sD[t] = dcur;
+      T dnext = T(0), unext[NB];
+#pragma unroll
+      for (int q = 0; q < NB; ++q) unext[q] = T(0);
+      if (kn >= 0) {
+        dnext = load_d(kn);
+        load_u(kn, unext);
       }
       __syncthreads();
       if (t < r) {
         T x[NB];
 #pragma unroll
-        for (int i = 0; i < NB; ++i) x[i] = i < w ? sX[i * kBatchMaxRhs + t] : T(0);
+        for (int i = 0; i < NB; ++i) x[i] = i < w ? sY[(kb + i) * r + t] : T(0);
 #pragma unroll
         for (int i = NB - 1; i >= 0; --i) {
           if (i < w) {
@@ -344,26 +362,30 @@ struct BatchSolver {
         }
 #pragma unroll
         for (int i = 0; i < NB; ++i)
-          if (i < w) {
-            sX[i * kBatchMaxRhs + t] = x[i];
-            M(kb + i, n + t) = x[i];
-          }
+          if (i < w) sY[(kb + i) * r + t] = x[i];
       }
       __syncthreads();
-      for (int i = t; i < kb; i += 256) {
-        T u[NB];
-#pragma unroll
-        for (int q = 0; q < NB; ++q) u[q] = q < w ? M(i, kb + q) : T(0);
+      if (t < kb) {
         for (int c = 0; c < r; ++c) {
-          T acc = M(i, n + c);
+          T acc = sY[t * r + c];
 #pragma unroll
           for (int q = 0; q < NB; ++q)
-            if (q < w) acc -= u[q] * sX[q * kBatchMaxRhs + c];
-          M(i, n + c) = acc;
+            if (q < w) acc -= ucur[q] * sY[(kb + q) * r + c];
+          sY[t * r + c] = acc;
         }
       }
+      if (kn < 0) break;
       __syncthreads();
+      dcur = dnext;
+#pragma unroll
+      for (int q = 0; q < NB; ++q) ucur[q] = unext[q];
+      kb = kn;
     }
+    __syncthreads();
+    for (int e = t; e < n * r; e += 256) M(e / r, n + e % r) = sY[e];
+    // The caller reads M with another thread-to-element mapping (the copy out
+    // of LDS), and the next system reuses sY.
+    __syncthreads();
   }
 };
 
