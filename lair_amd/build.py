@@ -8,6 +8,8 @@ feature; only the device pass uses it.
 import os
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -28,11 +30,17 @@ def hipcc():
     return "hipcc"
 
 
+FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
+
+
 def command(out=LIB, extra=()):
+    """One hipcc line building the whole library (kept for reference and for
+    callers that want a single command); build() compiles the sources in
+    parallel instead.  Every translation unit holds its own kernels and their
+    launchers, so no device-code linking across units is needed."""
     srcs = [os.path.join(HERE, "csrc", s) for s in SOURCES]
-    return [hipcc(), "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-shared",
-            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            "-Wno-unused-result", *extra, "-o", out, *srcs]
+    return [hipcc(), *FLAGS, "-shared", *extra, "-o", out, *srcs]
 
 
 def gen_command(out=GEN_LIB):
@@ -52,10 +60,29 @@ def _run(cmd, tmp, final, verbose):
     os.replace(tmp, final)
 
 
+def _compile(src, obj, verbose):
+    res = subprocess.run([hipcc(), *FLAGS, "-c", "-o", obj, src], capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"hipcc failed compiling {os.path.basename(src)}")
+    if verbose:
+        sys.stderr.write(res.stderr)
+
+
 def build(verbose=False):
+    """Compiles every source of the library in parallel, links it, then builds
+    the input-generator library."""
     os.makedirs(LIB_DIR, exist_ok=True)
-    _run(command(LIB + ".tmp"), LIB + ".tmp", LIB, verbose)
-    _run(gen_command(GEN_LIB + ".tmp"), GEN_LIB + ".tmp", GEN_LIB, verbose)
+    jobs = max(1, min(len(SOURCES) + 1, int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with tempfile.TemporaryDirectory(prefix="lair_build_") as tmp:
+        srcs = [os.path.join(HERE, "csrc", s) for s in SOURCES]
+        objs = [os.path.join(tmp, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+        with ThreadPoolExecutor(jobs) as pool:
+            gen = pool.submit(_run, gen_command(GEN_LIB + ".tmp"), GEN_LIB + ".tmp", GEN_LIB, verbose)
+            for f in [pool.submit(_compile, s, o, verbose) for s, o in zip(srcs, objs)]:
+                f.result()
+            gen.result()
+        _run([hipcc(), *FLAGS, "-shared", "-o", LIB + ".tmp", *objs], LIB + ".tmp", LIB, verbose)
     return LIB
 
 
