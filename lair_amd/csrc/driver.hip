@@ -968,7 +968,9 @@ struct lair_batched_plan {
   int64_t *ipiv = nullptr;
   int *info = nullptr;
   void *meta = nullptr;  // one device allocation for the metadata arrays
-  lair::BatchLaunch bl;
+  // One persistent launch per size class, largest systems first; each class
+  // has its own launch shape.
+  std::vector<lair::BatchLaunch> launches;
   size_t a_elems = 0, b_elems = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1045,9 +1047,11 @@ lair_status lair_gesv_batched_plan(lair_context *ctx, int32_t precision, int32_t
   }
   pl->a_elems = (size_t)sa;
   pl->b_elems = (size_t)sb;
+  // Largest systems first (by order, then right-hand sides), so that the
+  // queue hands out the long systems early and a size class is a prefix.
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-    return (int64_t)n[x] * n[x] * (n[x] + nrhs[x]) > (int64_t)n[y] * n[y] * (n[y] + nrhs[y]);
+    return n[x] != n[y] ? n[x] > n[y] : nrhs[x] > nrhs[y];
   });
   const size_t bytes = sizeof(int) * 2 * count + sizeof(int64_t) * 3 * count + sizeof(int) * count + 64;
   LAIR_HIP(hipMalloc(&pl->meta, bytes));
@@ -1082,19 +1086,21 @@ lair_status lair_gesv_batched_plan(lair_context *ctx, int32_t precision, int32_t
     if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) ncu = prop.multiProcessorCount;
   }
   batched_prepare();
-  std::vector<lair::BatchLaunch> cands = batched_candidates(bl, esize, ncu);
-  lair::BatchLaunch best = cands.front();
+  // Size classes: without the autotuner, one class with the first candidate
+  // shape.  With it, every threshold T splits the sorted systems into n > T
+  // and n <= T; each class gets its fastest candidate, timed on scratch copies
+  // of the inputs (the solve is in place), and the split with the smallest
+  // total wins (T = 256: no split).
+  std::vector<lair::BatchLaunch> best_plan{batched_candidates(bl, esize, ncu).front()};
   if ((flags & LAIR_BATCHED_AUTOTUNE) && count > 0) {
-    // Time every candidate on scratch copies (the solve is in place).
     void *sa_d = nullptr, *sb_d = nullptr;
     LAIR_HIP(hipMalloc(&sa_d, esize * pl->a_elems));
     LAIR_HIP(hipMalloc(&sb_d, esize * pl->b_elems));
     hipEvent_t e0, e1;
     LAIR_HIP(hipEventCreate(&e0));
     LAIR_HIP(hipEventCreate(&e1));
-    float best_ms = 1e30f;
-    for (const auto &c : cands) {
-      float cand_ms = 1e30f;
+    lair_status st = LAIR_OK;
+    auto time_launch = [&](const lair::BatchLaunch &c, float *best_ms) -> lair_status {
       for (int rep = 0; rep < 2; ++rep) {
         LAIR_HIP(hipMemcpyAsync(sa_d, a, esize * pl->a_elems, hipMemcpyDeviceToDevice, ctx->stream));
         LAIR_HIP(hipMemcpyAsync(sb_d, b, esize * pl->b_elems, hipMemcpyDeviceToDevice, ctx->stream));
@@ -1104,23 +1110,70 @@ lair_status lair_gesv_batched_plan(lair_context *ctx, int32_t precision, int32_t
         LAIR_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
         LAIR_HIP(hipEventElapsedTime(&ms, e0, e1));
-        cand_ms = std::min(cand_ms, ms);
+        *best_ms = std::min(*best_ms, ms);
       }
-      if (cand_ms < best_ms) {
-        best_ms = cand_ms;
-        best = c;
+      return LAIR_OK;
+    };
+    // The fastest candidate for the systems order[first, first + cnt).
+    auto tune_class = [&](int first, int cnt, int queue_slot, lair::BatchLaunch *out,
+                          float *out_ms) -> lair_status {
+      lair::BatchLaunch base = bl;
+      base.count = cnt;
+      base.order = d_order + first;
+      base.queue = d_queue + queue_slot;
+      *out_ms = 1e30f;
+      for (const auto &c : batched_candidates(base, esize, ncu)) {
+        float ms = 1e30f;
+        lair_status s2 = time_launch(c, &ms);
+        if (s2 != LAIR_OK) return s2;
+        if (ms < *out_ms) {
+          *out_ms = ms;
+          *out = c;
+        }
+      }
+      return LAIR_OK;
+    };
+    float best_total = 1e30f;
+    for (int T : {256, 192, 144}) {
+      int split = 0;  // systems with n > T (a prefix of the order)
+      while (split < count && n[order[split]] > T) ++split;
+      if (T != 256 && (split == 0 || split == count)) continue;
+      std::vector<lair::BatchLaunch> cand_plan;
+      float total = 0.f;
+      if (split > 0) {
+        lair::BatchLaunch la;
+        float ms;
+        st = tune_class(0, split, 0, &la, &ms);
+        if (st != LAIR_OK) break;
+        cand_plan.push_back(la);
+        total += ms;
+      }
+      if (split < count) {
+        lair::BatchLaunch lb;
+        float ms;
+        st = tune_class(split, count - split, 1, &lb, &ms);
+        if (st != LAIR_OK) break;
+        cand_plan.push_back(lb);
+        total += ms;
+      }
+      if (total < best_total) {
+        best_total = total;
+        best_plan = cand_plan;
       }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     hipFree(sa_d);
     hipFree(sb_d);
+    if (st != LAIR_OK) return st;
   }
-  pl->bl = best;
-  // Capture the (queue reset, persistent kernel) pair.
+  pl->launches = best_plan;
+  // Capture the (queue reset, persistent kernel) pair of every class.
   LAIR_HIP(hipStreamSynchronize(ctx->stream));
   LAIR_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-  hipError_t le = batched_launch(pl.get(), pl->bl, a, b);
+  hipError_t le = hipSuccess;
+  for (const auto &l : pl->launches)
+    if (le == hipSuccess) le = batched_launch(pl.get(), l, a, b);
   hipGraph_t g = nullptr;
   hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
   if (le != hipSuccess) return hip_status(le, "batched capture");
@@ -1142,9 +1195,10 @@ lair_status lair_gesv_batched_run(lair_batched_plan *plan) {
 lair_status lair_gesv_batched_config(lair_batched_plan *plan, int32_t *nb, int32_t *grid,
                                      int32_t *lds_elems) {
   if (!plan) return -1;
-  if (nb) *nb = plan->bl.nb;
-  if (grid) *grid = plan->bl.grid;
-  if (lds_elems) *lds_elems = plan->bl.lds_max_elems;
+  // The launch shape of the largest systems' class.
+  if (nb) *nb = plan->launches.front().nb;
+  if (grid) *grid = plan->launches.front().grid;
+  if (lds_elems) *lds_elems = plan->launches.front().lds_max_elems;
   return LAIR_OK;
 }
 
