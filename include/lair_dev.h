@@ -190,10 +190,11 @@ lair_status lair_dbackward_error(lair_context *ctx, int64_t n, int64_t nrhs, con
  * sum_{j<i} n_j nrhs_j of `b`; pivots of A_i (0-based, int64) at offset
  * sum_{j<i} n_j of `ipiv`; info[i] = -1 or the last zero-pivot column.
  * a is overwritten by the LU factors and b by X.  precision: 0 = FP64,
- * 1 = FP32 (a, b are float).  The plan sorts the systems by decreasing cost,
- * optionally autotunes the launch (flags & LAIR_BATCHED_AUTOTUNE: candidate
- * shapes are timed on scratch copies of a and b) and captures the launch in a
- * hipGraph on the context's stream. */
+ * 1 = FP32 (a, b are float).  The plan sorts the systems by decreasing n,
+ * optionally autotunes (flags & LAIR_BATCHED_AUTOTUNE: the systems may be
+ * split into two size classes, and each class's launch shape is chosen by
+ * timing candidates on scratch copies of a and b) and captures the launches
+ * in one hipGraph on the context's stream. */
 #define LAIR_BATCHED_AUTOTUNE 1
 typedef struct lair_batched_plan lair_batched_plan;
 lair_status lair_gesv_batched_plan(lair_context *ctx, int32_t precision, int32_t count,
@@ -202,7 +203,8 @@ lair_status lair_gesv_batched_plan(lair_context *ctx, int32_t precision, int32_t
                                    lair_batched_plan **plan);
 /* Replays the captured graph (asynchronous on the context's stream). */
 lair_status lair_gesv_batched_run(lair_batched_plan *plan);
-/* Launch shape chosen by the plan: panel width, workgroups, LDS budget (elements). */
+/* Launch shape chosen for the largest systems: panel width, workgroups, LDS
+ * budget (elements). */
 lair_status lair_gesv_batched_config(lair_batched_plan *plan, int32_t *nb, int32_t *grid,
                                      int32_t *lds_elems);
 lair_status lair_gesv_batched_destroy(lair_batched_plan *plan);
