@@ -91,6 +91,12 @@ __device__ __forceinline__ double bwave_max(double v) {
   return __hiloint2double(hi, lo);
 }
 
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename T, int NB>
 struct BatchSolver {
   int t, lane, wave;
@@ -128,6 +134,73 @@ struct BatchSolver {
     T v[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c) v[c] = (has && c < w) ? M(row, kb + c) : T(0);
+    if (n - kb <= 64) {
+      // Every remaining row belongs to wave 0, which runs the column loop
+      // alone: the same argmax, tie rule and arithmetic, with a wavefront
+      // fence in place of the workgroup barrier and no cross-wave reduction.
+      if (wave == 0) {
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+          if (j < w) {
+            const int d = kb + j;
+            const int par = j & 1;
+            T *prow = scand + par * 4 * NB;
+            T *diag = sdiag + par * NB;
+            int *srcs = s_srcx + par * 5;
+            const double x = (has && row >= d) ? fabs((double)v[j]) : 0.0;
+            const int xr = (has && row >= d && x > 0.0) ? row : INT_MAX;
+            const double vmax = bwave_max(xr != INT_MAX ? x : 0.0);
+            unsigned long long mm = __ballot(xr != INT_MAX && x == vmax);
+            int wr = INT_MAX;
+            while (mm) {
+              const int l = __ffsll((long long)mm) - 1;
+              mm &= mm - 1;
+              const int rl = __builtin_amdgcn_readlane(xr, l);
+              wr = rl < wr ? rl : wr;
+            }
+            const bool singular = (wr == INT_MAX);
+            const int pr = singular ? d : wr;
+            if (row == pr) {
+#pragma unroll
+              for (int c = 0; c < NB; ++c) prow[c] = v[c];
+              srcs[0] = src;
+            }
+            if (row == d) {
+#pragma unroll
+              for (int c = 0; c < NB; ++c) diag[c] = v[c];
+              srcs[4] = src;
+            }
+            wave_sync();
+            if (pr != d) {
+              if (row == d) {
+#pragma unroll
+                for (int c = 0; c < NB; ++c) v[c] = prow[c];
+                src = srcs[0];
+              } else if (row == pr) {
+#pragma unroll
+                for (int c = 0; c < NB; ++c) v[c] = diag[c];
+                src = srcs[4];
+              }
+            }
+            if (lane == 0) {
+              spiv[j] = pr;
+              if (singular) *sinfo = d;
+            }
+            if (!singular && has && row > d) {
+              const T recip = T(1) / prow[j];
+              const T l = v[j] * recip;
+              v[j] = l;
+#pragma unroll
+              for (int c = j + 1; c < NB; ++c) {
+                const T e = l * prow[c];
+                v[c] = v[c] - e;
+              }
+            }
+            wave_sync();
+          }
+        }
+      }
+    } else {
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
       if (j < w) {
@@ -204,6 +277,7 @@ struct BatchSolver {
           }
         }
       }
+    }
     }
     if (has) {
 #pragma unroll
