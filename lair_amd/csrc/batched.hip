@@ -1,8 +1,8 @@
 // Batched solve of many independent small systems A_i X_i = B_i (n_i <= 256,
-// nrhs_i <= 16) in one persistent launch.
+// nrhs_i <= 16) in persistent launches, one per size class of the plan.
 //
 // Workgroups pull system indices from an atomic queue (an order sorted by
-// decreasing n^3, so the largest systems start first) until the queue is
+// decreasing n, so the largest systems start first) until the queue is
 // empty; every workgroup reaches that exit.  Each system is factored as the
 // augmented matrix [A | B] with a blocked right-looking LU with partial
 // pivoting (src/lapack/getrf.rs:46-120, blocked; the interchanges are applied
@@ -14,7 +14,7 @@
 //    per wave and a fixed-order reduction of the four wave winners (ties ->
 //    lowest row; NaN and zero never chosen, src/blas/iamax.rs:6-21), the
 //    interchange, the reciprocal scaling and the rank-1 update
-//    (getrf.rs:76-88);
+//    (getrf.rs:76-88); once the remaining rows fit one wave, wave 0 alone;
 //  * the panel's interchanges form one row map, applied to the other columns
 //    of [A | B] as a gather;
 //  * U12 = L11^-1 A12 column by column, then A22 -= L21 U12 with L21 and U12
