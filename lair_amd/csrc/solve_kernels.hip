@@ -162,20 +162,42 @@ template hipError_t launch_rowswap<float>(lair_context *, float *, int64_t, int,
                                           const int64_t *, int, int, int *);
 
 // perm from the full pivot vector (for factors supplied by the caller):
-// one thread applies the n interchanges to an LDS index array.
+// one thread applies the n interchanges to an LDS index array.  When the
+// pivots fit beside it (n <= kPermPivLdsMax) they are staged in LDS first, so
+// the serial loop touches LDS only.  A pivot outside [0, n) is skipped rather
+// than followed out of bounds.
+constexpr int kPermPivLdsMax = 20480;  // 2 n ints in 160 KB
 __global__ void __launch_bounds__(256) perm_from_ipiv_kernel(const int64_t *ipiv, int n, int *perm) {
-  extern __shared__ int sp[];  // [n] when it fits, else unused
+  extern __shared__ int sp[];  // [n] perm when it fits, then [n] pivots when both fit
   const bool lds = n <= kPermLdsMax;
+  const bool lds_piv = n <= kPermPivLdsMax;
   int *p = lds ? sp : perm;
-  for (int i = threadIdx.x; i < n; i += 256) p[i] = i;
+  int *piv = sp + n;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    p[i] = i;
+    if (lds_piv) piv[i] = (int)ipiv[i];
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int i = 0; i < n; ++i) {
-      const int q = (int)ipiv[i];
-      if (q != i) {
-        const int tmp = p[i];
-        p[i] = p[q];
-        p[q] = tmp;
+    if (lds_piv) {
+      int q = piv[0];
+      for (int i = 0; i < n; ++i) {
+        const int qn = i + 1 < n ? piv[i + 1] : 0;  // next pivot, read ahead
+        if (q != i && (unsigned)q < (unsigned)n) {
+          const int tmp = p[i];
+          p[i] = p[q];
+          p[q] = tmp;
+        }
+        q = qn;
+      }
+    } else {
+      for (int i = 0; i < n; ++i) {
+        const int q = (int)ipiv[i];
+        if (q != i && (unsigned)q < (unsigned)n) {
+          const int tmp = p[i];
+          p[i] = p[q];
+          p[q] = tmp;
+        }
       }
     }
   }
@@ -186,7 +208,8 @@ __global__ void __launch_bounds__(256) perm_from_ipiv_kernel(const int64_t *ipiv
 
 hipError_t launch_perm_from_ipiv(lair_context *ctx, const int64_t *ipiv, int n, int *perm) {
   if (n <= 0) return hipSuccess;
-  const size_t shm = n <= kPermLdsMax ? (size_t)n * sizeof(int) : 0;
+  const size_t shm = n <= kPermPivLdsMax ? 2 * (size_t)n * sizeof(int)
+                                         : (n <= kPermLdsMax ? (size_t)n * sizeof(int) : 0);
   static bool once = (allow_large_lds(perm_from_ipiv_kernel), true);
   (void)once;
   perm_from_ipiv_kernel<<<1, 256, shm, ctx->stream>>>(ipiv, n, perm);
