@@ -87,6 +87,31 @@ def test_batched_matches_oracle(ctx, dtype, autotune):
         assert be.max() <= n * eps, (i, n, be.max())
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_batched_extreme_shapes(ctx, dtype):
+    """The limits of the batched path: n in {1, 2, 255, 256} with up to 16
+    right-hand sides (at n = 256 and 16 of them, Y fills its whole LDS region
+    in the back substitution), mixed with mid sizes."""
+    rng = np.random.default_rng(41 if dtype == np.float64 else 42)
+    ns = np.array([256, 256, 255, 1, 2, 256, 100, 1, 255, 64, 2, 17], np.int32)
+    rs = np.array([16, 1, 16, 16, 16, 8, 16, 1, 3, 16, 5, 16], np.int32)
+    As = [(rng.standard_normal((n, n)) + n * np.eye(n)).astype(dtype) for n in ns]
+    Bs = [rng.standard_normal((n, r)).astype(dtype) for n, r in zip(ns, rs)]
+    eps = np.finfo(dtype).eps
+    for autotune in (True, False):
+        outs, _ = run_batch(ctx, ns, rs, As, Bs, dtype, autotune=autotune)
+        lu, x, piv, info = outs[0]
+        assert (info == -1).all()
+        for i, (lui, xi, pi) in enumerate(split(ns, rs, lu, x, piv)):
+            ref = As[i].copy()
+            piv_o, _ = O.getrf(ref) if dtype == np.float64 else O.sgetrf(ref)
+            np.testing.assert_array_equal(pi, piv_o, err_msg=f"system {i} n={ns[i]}")
+            a64 = As[i].astype(np.float64)
+            r = Bs[i].astype(np.float64) - a64 @ xi.astype(np.float64)
+            be = np.abs(r).max(0) / (np.abs(a64).sum(1).max() * np.abs(xi).max(0))
+            assert be.max() <= ns[i] * eps, (autotune, i, ns[i], rs[i], be.max())
+
+
 def test_batched_graph_replays_are_identical(ctx):
     ns, rs, As, Bs = make_batch(200, 21, np.float64, 1, 200)
     outs, _ = run_batch(ctx, ns, rs, As, Bs, np.float64, replays=3)
