@@ -215,6 +215,33 @@ def test_getrs_trans_and_multi_rhs(n, nrhs):
     np.testing.assert_allclose(y, yo, rtol=0, atol=1e-9 * np.abs(yo).max())
 
 
+@pytest.mark.parametrize("n", [20500, 33000])
+def test_getrs_large_caller_pivots_permutation(n):
+    """getrs from caller-supplied factors composes its row permutation from the
+    pivots (laswp.rs:21-39 order).  With identity factors the solve returns
+    P b exactly, so the composition is checked beyond the sizes whose pivots
+    (n > 20480) or permutation (n > 32768) still fit LDS."""
+    rng = np.random.default_rng(n)
+    piv = np.array([rng.integers(i, n) for i in range(n)], np.int64)
+    b = rng.standard_normal((n, 1)).astype(np.float32)
+    expect = b.copy()
+    for i, q in enumerate(piv):  # getrs.rs:23, laswp.rs:28-38
+        if q != i:
+            expect[[i, q]] = expect[[q, i]]
+    ctx = L.Context()
+    lu = L.DeviceArray.from_host(np.eye(n, dtype=np.float32))
+    dpiv = L.DeviceArray.from_host(piv.reshape(-1, 1))
+    db = L.DeviceArray.from_host(b)
+    try:
+        L.check(ctx.lib.lair_sgetrs_device(ctx.h, n, 1, lu.ptr, n, dpiv.ptr, db.ptr, 1, 0), "sgetrs")
+        ctx.synchronize()
+        np.testing.assert_array_equal(db.download(), expect)
+    finally:
+        for d in (lu, dpiv, db):
+            d.free()
+        ctx.close()
+
+
 def test_singular_last_zero_pivot_wins():  # getrf.rs:72-73
     a = np.array([[0., 0., 1.], [0., 0., 2.], [0., 0., 3.]])
     ref = a.copy()
