@@ -583,7 +583,10 @@ hipError_t launch_trsv_dataflow(lair_context *ctx, const T *lu, int64_t lda, con
                                 bool forward, bool trans) {
   if (n <= 0 || nrhs <= 0) return hipSuccess;
   constexpr int BS = TrsvBlock<T>::BS;
-  const int rc = nrhs == 1 ? 1 : (nrhs <= 4 ? 4 : 16);
+  // Right-hand sides per workgroup: 4 columns keep each chain link short
+  // (the link's products and reductions grow with the chunk width), and the
+  // chunks run as independent chains.
+  const int rc = nrhs == 1 ? 1 : 4;
   const int nblk = (n + BS - 1) / BS;
   const int nchunks = (nrhs + rc - 1) / rc;
   hipError_t e = ensure(ctx->ws_flags, sizeof(uint32_t) * (size_t)nblk * nchunks);
@@ -615,16 +618,13 @@ hipError_t launch_trsv_dataflow(lair_context *ctx, const T *lu, int64_t lda, con
   const unsigned grid = (unsigned)(nblk * nchunks);
   const size_t shm = (size_t)BS * (BS + 16 / sizeof(T)) * sizeof(T);
   static bool once = (allow_large_lds(trsv_dataflow_kernel<T, BS, 1>),
-                      allow_large_lds(trsv_dataflow_kernel<T, BS, 4>),
-                      allow_large_lds(trsv_dataflow_kernel<T, BS, 16>), true);
+                      allow_large_lds(trsv_dataflow_kernel<T, BS, 4>), true);
   (void)once;
   const int slot = prof_begin(ctx, LAIR_KCLASS_GETRS);
   if (rc == 1)
     trsv_dataflow_kernel<T, BS, 1><<<grid, 256, shm, ctx->stream>>>(a);
-  else if (rc == 4)
-    trsv_dataflow_kernel<T, BS, 4><<<grid, 256, shm, ctx->stream>>>(a);
   else
-    trsv_dataflow_kernel<T, BS, 16><<<grid, 256, shm, ctx->stream>>>(a);
+    trsv_dataflow_kernel<T, BS, 4><<<grid, 256, shm, ctx->stream>>>(a);
   prof_end(ctx, slot, LAIR_KCLASS_GETRS, (double)n * n * nrhs,
            sizeof(T) * (0.5 * (double)n * n + 2.0 * n * nrhs));
   return hipGetLastError();
