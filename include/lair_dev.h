@@ -62,8 +62,21 @@ typedef struct lair_context lair_context;
  * mutex-guarded default context when `ctx` is NULL. */
 lair_status lair_context_create(lair_context **ctx);
 lair_status lair_context_destroy(lair_context *ctx);
-/* Blocks until all work queued on the context's stream has finished. */
+/* Blocks until all work queued on the context's stream has finished, then
+ * reports what the context's device status word recorded since the last
+ * synchronizing call: LAIR_ERR_SYNC when a cooperative kernel's bounded wait
+ * gave up, LAIR_ERR_SHAPE when a caller-supplied pivot was outside [0, n).
+ * The word is sticky until a synchronizing call (this one, or any call
+ * documented as synchronizing) reads and resets it; while it is set, the
+ * cooperative kernels give up their waits at once.  No reference analogue. */
 lair_status lair_context_synchronize(lair_context *ctx);
+/* Polls a cooperative kernel's bounded wait makes before it gives up
+ * (default 8388608, about seconds).  0 makes every wait give up at once, so
+ * that the calls that use such kernels (getrf of more than one column
+ * panel, getrs and the adjoint for n > 128, the mixed solver) return
+ * LAIR_ERR_SYNC deterministically: used to test the failure path.  No
+ * reference analogue. */
+lair_status lair_context_set_wait_limit(lair_context *ctx, int64_t polls);
 /* The HIP stream (hipStream_t) the context launches on, as an opaque pointer. */
 void *lair_context_stream(lair_context *ctx);
 /* Human-readable message for the last error seen on this thread. */
@@ -136,7 +149,9 @@ lair_status lair_sgetrf_pivot_device(lair_context *ctx, int64_t m, int64_t n, fl
                                      int64_t *singular);
 
 /* getrs on device data: b is n x nrhs row-major (leading dimension ldb),
- * overwritten with the solution of A X = B (trans = 0) or A^T X = B. */
+ * overwritten with the solution of A X = B (trans = 0) or A^T X = B.  The
+ * pivots are range-checked on the device; the call synchronizes and returns
+ * LAIR_ERR_SHAPE when one is outside [0, n), as the host-pointer getrs does. */
 lair_status lair_dgetrs_device(lair_context *ctx, int64_t n, int64_t nrhs, const double *lu,
                                int64_t lda, const int64_t *ipiv, double *b, int64_t ldb,
                                int32_t trans);

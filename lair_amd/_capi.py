@@ -51,6 +51,7 @@ SIGNATURES = [
     ("lair_context_create", i32, [ctypes.POINTER(vp)]),
     ("lair_context_destroy", i32, [vp]),
     ("lair_context_synchronize", i32, [vp]),
+    ("lair_context_set_wait_limit", i32, [vp, i64]),
     ("lair_context_stream", vp, [vp]),
     ("lair_last_error", ctypes.c_char_p, []),
     ("lair_malloc", i32, [ctypes.POINTER(vp), ctypes.c_uint64]),

@@ -10,6 +10,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -108,6 +109,29 @@ static lair_status hip_status(hipError_t e, const char *what) {
   return e == hipErrorOutOfMemory ? LAIR_ERR_NOMEM : LAIR_ERR_DEVICE;
 }
 
+// Synchronizes the context's stream and reports what the device status word
+// recorded since the last check: a caller pivot outside [0, n) as
+// LAIR_ERR_SHAPE, a bounded wait that gave up as LAIR_ERR_SYNC.  The word is
+// sticky (kernels that see it give up their own waits at once) until it is
+// read here and reset.
+constexpr int32_t kErrPivotBit = 2;  // device_util.h kErrPivot
+static lair_status sync_status(lair_context *ctx) {
+  int32_t err = 0;
+  hipError_t e = hipMemcpyAsync(&err, ctx->err, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_status(e, "synchronize");
+  if (!err) return LAIR_OK;
+  e = launch_fill_u32(ctx, reinterpret_cast<uint32_t *>(ctx->err), 1, 0u);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_status(e, "reset");
+  if (err & kErrPivotBit) {
+    set_error("pivot index out of range");
+    return LAIR_ERR_SHAPE;
+  }
+  set_error("a cooperative kernel's bounded wait gave up (LAIR_ERR_SYNC)");
+  return LAIR_ERR_SYNC;
+}
+
 // Blocked right-looking LU with partial pivoting (the panel -> laswp -> trsm
 // -> gemm sequence of the reference's recursive_inner, getrf.rs:216-322, with
 // a fixed panel width nb).  Each panel of nb columns is factored as leaves of
@@ -153,6 +177,36 @@ static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a,
   }
   // Factors the panel of columns [k, k + kb) on the context's stream.
   auto factor_panel = [&](int k, int kb) -> hipError_t {
+    // Resident leaves (rows kept in LDS for the whole leaf).  A leaf narrower
+    // than the panel is followed by its interchanges on the panel's other
+    // columns, the U rows right of it (trsm) and the update below (gemm).
+    const int lw = tournament ? 0 : rpanel_leaf_width<T>(ctx, kb, m - k);
+    if (lw > 0) {
+      for (int c = k; c < k + kb; c += lw) {
+        const int w = std::min(lw, k + kb - c);
+        hipError_t e2 = launch_rpanel<T>(ctx, a, lda, c, m, c, w, lw, ipiv);
+        if (e2 != hipSuccess) return e2;
+        const int npv = std::min(w, m - c);
+        if (npv <= 0) continue;
+        if (c > k) {
+          e2 = launch_rowswap<T>(ctx, a, lda, k, c, k, k, ipiv, c, c + npv, nullptr);
+          if (e2 != hipSuccess) return e2;
+        }
+        const int c1 = c + w, rest = k + kb - c1;
+        if (rest > 0) {
+          e2 = launch_rowswap<T>(ctx, a, lda, c1, k + kb, c1, c1, ipiv, c, c + npv, nullptr);
+          if (e2 == hipSuccess)
+            e2 = launch_trsm_llu<T>(ctx, npv, rest, a + (int64_t)c * lda + c, lda, a + (int64_t)c * lda + c1,
+                                    lda);
+          if (e2 == hipSuccess && m > c + npv)
+            e2 = launch_gemm_minus<T>(ctx, mode, m - (c + npv), rest, npv, a + (int64_t)(c + npv) * lda + c,
+                                      lda, false, a + (int64_t)c * lda + c1, lda,
+                                      a + (int64_t)(c + npv) * lda + c1, lda, LAIR_KCLASS_GEMM_PANEL);
+          if (e2 != hipSuccess) return e2;
+        }
+      }
+      return hipSuccess;
+    }
     for (int c = k; c < k + kb; c += kLeafW) {
       const int w = std::min(kLeafW, k + kb - c);
       // The leaf kernel also applies its interchanges to the other columns
@@ -227,14 +281,8 @@ static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a,
     LAIR_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_bulk, 0));
   }
   if (singular) {
-    int32_t err = 0;
     LAIR_HIP(hipMemcpyAsync(singular, ctx->sing, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-    LAIR_HIP(hipMemcpyAsync(&err, ctx->err, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-    LAIR_HIP(hipStreamSynchronize(ctx->stream));
-    if (err) {
-      set_error("cooperative panel wait expired");
-      return LAIR_ERR_SYNC;
-    }
+    return sync_status(ctx);
   }
   return LAIR_OK;
 }
@@ -277,10 +325,8 @@ static lair_status tune_nb(lair_context *ctx, int64_t m, int64_t n, const T *a, 
                        nullptr);
     if (st != LAIR_OK) break;
     hipEventRecord(e1, ctx->stream);
-    if (hipEventSynchronize(e1) != hipSuccess) {
-      st = hip_status(hipErrorUnknown, "autotune");
-      break;
-    }
+    st = sync_status(ctx);
+    if (st != LAIR_OK) break;
     float ms = 0.f;
     hipEventElapsedTime(&ms, e0, e1);
     if (ms < best_ms) {
@@ -407,7 +453,8 @@ static lair_status backward_error_from(lair_context *ctx, int n, int nrhs, const
   if (e != hipSuccess) return hip_status(e, "colmax");
   std::vector<unsigned long long> h(1 + 2 * (size_t)nrhs);
   LAIR_HIP(hipMemcpyAsync(h.data(), norms, h.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
-  LAIR_HIP(hipStreamSynchronize(ctx->stream));
+  lair_status st = sync_status(ctx);
+  if (st != LAIR_OK) return st;
   const double na = bits_to_double(h[0]);
   double worst = 0.0;
   for (int c = 0; c < nrhs; ++c) {
@@ -483,7 +530,8 @@ static lair_status host_getrf(int64_t m, int64_t n, T *a, int64_t rs, int64_t cs
   if (st != LAIR_OK) return st;
   LAIR_HIP(hipMemcpyAsync(h.data(), da, sizeof(T) * m * ld, hipMemcpyDeviceToHost, ctx->stream));
   LAIR_HIP(hipMemcpyAsync(ipiv, dp, sizeof(int64_t) * kmin, hipMemcpyDeviceToHost, ctx->stream));
-  LAIR_HIP(hipStreamSynchronize(ctx->stream));
+  st = sync_status(ctx);
+  if (st != LAIR_OK) return st;
   scatter(m, n, h.data(), ld, a, rs, cs);
   return LAIR_OK;
 }
@@ -520,7 +568,8 @@ static lair_status host_getrs(int64_t n, int64_t nrhs, const T *lu, int64_t lrs,
   lair_status st = getrs_impl<T>(ctx, n, nrhs, da, ld, dp, db, nrhs, trans);
   if (st != LAIR_OK) return st;
   LAIR_HIP(hipMemcpyAsync(hb.data(), db, sizeof(T) * n * nrhs, hipMemcpyDeviceToHost, ctx->stream));
-  LAIR_HIP(hipStreamSynchronize(ctx->stream));
+  st = sync_status(ctx);
+  if (st != LAIR_OK) return st;
   scatter(n, nrhs, hb.data(), nrhs, b, brs, bcs);
   return LAIR_OK;
 }
@@ -539,6 +588,8 @@ lair_status lair_context_create(lair_context **out) {
   LAIR_HIP(hipMalloc(&ctx->drec, sizeof(unsigned long long) * 2 * 64));
   LAIR_HIP(hipMalloc(&ctx->crec, sizeof(unsigned long long) * kCaluWords));
   LAIR_HIP(hipMalloc(&ctx->err, sizeof(int32_t)));
+  LAIR_HIP(hipMalloc(&ctx->rp_hdr, sizeof(unsigned long long) * 2 * kMaxPanelWG * 2));
+  LAIR_HIP(hipMalloc(&ctx->rp_rowd, sizeof(unsigned long long) * 2 * kMaxPanelWG * kRpRowWords));
   LAIR_HIP(hipMalloc(&ctx->sing, sizeof(int64_t)));
   LAIR_HIP(hipMalloc(&ctx->swap_rows, sizeof(int) * kMaxSwapRows));
   LAIR_HIP(hipMalloc(&ctx->swap_src, sizeof(int) * kMaxSwapRows));
@@ -561,7 +612,12 @@ lair_status lair_context_create(lair_context **out) {
   LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->rec), 4 * kMaxPanelWG * 64, 0u));
   LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->drec), 4 * 64, 0u));
   LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->crec), 2 * kCaluWords, 0u));
+  LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->rp_hdr), 2 * 2 * kMaxPanelWG * 2, 0u));
+  LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->rp_rowd),
+                           2 * 2 * kMaxPanelWG * kRpRowWords, 0u));
   ctx->seq = 1;
+  ctx->rp_seq = 1;
+  if (const char *ev = std::getenv("LAIR_PANEL_MAX_WG")) ctx->rp_max_wg = std::max(1, std::atoi(ev));
   LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->err), 1, 0u));
   LAIR_HIP(hipStreamSynchronize(ctx->stream));
   *out = ctx.release();
@@ -576,6 +632,8 @@ lair_status lair_context_destroy(lair_context *ctx) {
   hipFree(ctx->drec);
   hipFree(ctx->crec);
   hipFree(ctx->err);
+  hipFree(ctx->rp_hdr);
+  hipFree(ctx->rp_rowd);
   hipFree(ctx->sing);
   hipFree(ctx->norms);
   hipFree(ctx->swap_rows);
@@ -603,7 +661,15 @@ lair_status lair_context_destroy(lair_context *ctx) {
 lair_status lair_context_synchronize(lair_context *ctx) {
   if (!ctx) ctx = default_ctx();
   if (!ctx) return LAIR_ERR_DEVICE;
-  LAIR_HIP(hipStreamSynchronize(ctx->stream));
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return sync_status(ctx);
+}
+
+lair_status lair_context_set_wait_limit(lair_context *ctx, int64_t polls) {
+  if (!ctx) return -1;
+  if (polls < 0 || polls > INT32_MAX) return -2;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->wait_limit = (int32_t)polls;
   return LAIR_OK;
 }
 
@@ -677,7 +743,8 @@ lair_status lair_dgetrs_device(lair_context *ctx, int64_t n, int64_t nrhs, const
                                int32_t trans) {
   if (!ctx) return -1;
   std::lock_guard<std::mutex> lk(ctx->mu);
-  return getrs_impl<double>(ctx, n, nrhs, lu, lda, ipiv, b, ldb, trans);
+  lair_status st = getrs_impl<double>(ctx, n, nrhs, lu, lda, ipiv, b, ldb, trans);
+  return st != LAIR_OK ? st : sync_status(ctx);
 }
 
 lair_status lair_sgetrs_device(lair_context *ctx, int64_t n, int64_t nrhs, const float *lu,
@@ -685,7 +752,8 @@ lair_status lair_sgetrs_device(lair_context *ctx, int64_t n, int64_t nrhs, const
                                int32_t trans) {
   if (!ctx) return -1;
   std::lock_guard<std::mutex> lk(ctx->mu);
-  return getrs_impl<float>(ctx, n, nrhs, lu, lda, ipiv, b, ldb, trans);
+  lair_status st = getrs_impl<float>(ctx, n, nrhs, lu, lda, ipiv, b, ldb, trans);
+  return st != LAIR_OK ? st : sync_status(ctx);
 }
 
 lair_status lair_dgetrf(int64_t m, int64_t n, double *a, int64_t rs, int64_t cs, int64_t *ipiv,
@@ -736,7 +804,8 @@ lair_status lair_dlaswp(int64_t ncols, double *a, int64_t rs, int64_t cs, int64_
   hipError_t e = launch_laswp<double>(ctx, da, nc, 0, (int)nc, 0, 0, dp, (int)begin, (int)npiv, false);
   if (e != hipSuccess) return hip_status(e, "laswp");
   LAIR_HIP(hipMemcpyAsync(h.data(), da, sizeof(double) * rows * nc, hipMemcpyDeviceToHost, ctx->stream));
-  LAIR_HIP(hipStreamSynchronize(ctx->stream));
+  lair_status st = sync_status(ctx);
+  if (st != LAIR_OK) return st;
   scatter(rows, nc, h.data(), nc, a, rs, cs);
   return LAIR_OK;
 }
@@ -806,8 +875,7 @@ lair_status lair_dgesv_device(lair_context *ctx, int64_t n, int64_t nrhs, const 
   std::lock_guard<std::mutex> lk(ctx->mu);
   lair_status st = dgesv_device_locked(ctx, n, nrhs, a, lda, b, ldb, x, ldx);
   if (st != LAIR_OK) return st;
-  LAIR_HIP(hipStreamSynchronize(ctx->stream));
-  return LAIR_OK;
+  return sync_status(ctx);
 }
 
 static lair_status berr_locked(lair_context *ctx, int n, int nrhs, const double *a, int64_t lda,
@@ -1234,8 +1302,7 @@ static lair_status solve_backward_locked(lair_context *ctx, int n, int nrhs, con
   prof_end(ctx, slot, LAIR_KCLASS_GEMM, 2.0 * n * (double)n * nrhs,
            sizeof(double) * ((double)n * n + 2.0 * n * nrhs));
   if (e != hipSuccess) return hip_status(e, "gemm");
-  LAIR_HIP(hipStreamSynchronize(ctx->stream));
-  return LAIR_OK;
+  return sync_status(ctx);
 }
 
 lair_status lair_dsolve_backward_device(lair_context *ctx, int64_t n, int64_t nrhs,
@@ -1278,8 +1345,7 @@ lair_status lair_dsolve_backward(int64_t n, int64_t nrhs, const double *lu, cons
   if (st != LAIR_OK) return st;
   LAIR_HIP(hipMemcpyAsync(bbar, d_bb, sizeof(double) * nr, hipMemcpyDeviceToHost, ctx->stream));
   LAIR_HIP(hipMemcpyAsync(abar, d_ab, sizeof(double) * nn, hipMemcpyDeviceToHost, ctx->stream));
-  LAIR_HIP(hipStreamSynchronize(ctx->stream));
-  return LAIR_OK;
+  return sync_status(ctx);
 }
 
 // ---- instrumentation -----------------------------------------------------

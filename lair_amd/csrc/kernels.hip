@@ -12,6 +12,7 @@
 
 #include <algorithm>
 
+#include "device_util.h"
 #include "lair_internal.h"
 
 namespace lair {
@@ -20,18 +21,6 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
-// Bounded wait: a cooperative panel gives up (and flags the error) after this
-// many polls instead of hanging when a peer never arrives.
-constexpr int kSpinLimit = 1 << 23;
-
-template <typename T>
-__device__ __forceinline__ void st_agent(T *p, T v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ T ld_agent(const T *p) {
-  return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 // (value, row) ordering of iamax: larger |x| wins, ties go to the lower row.
 // Candidates only exist for |x| > 0 (strict '>' against a running maximum that
@@ -103,78 +92,12 @@ struct PanelArgs {
   unsigned long long *drec;  // [2][kRecWords] tagged words of the diagonal row
   uint32_t seq;
   int32_t *err;
+  int wait_limit;  // polls before a wait gives up (0: every wait gives up at once)
   int stride;  // launched slots per workgroup; only slot 0 of each group works
   int rest1;   // panel columns [col0 + wcols, rest1) receive the leaf's
                // interchanges and their U rows (L11^-1 A12) in this launch;
   int left0;   // columns [left0, col0) receive the interchanges only
 };
-
-__device__ __forceinline__ unsigned long long tagged(uint32_t tag, uint32_t data) {
-  return ((unsigned long long)tag << 32) | data;
-}
-
-template <typename T>
-__device__ __forceinline__ uint32_t word_of(T x, int part) {
-  if constexpr (sizeof(T) == 8) {
-    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
-    return part == 0 ? (uint32_t)(b >> 32) : (uint32_t)b;
-  } else {
-    return (uint32_t)__float_as_uint(x);
-  }
-}
-
-template <typename T>
-__device__ __forceinline__ T from_words(const uint32_t *w) {
-  if constexpr (sizeof(T) == 8)
-    return __longlong_as_double((long long)(((unsigned long long)w[0] << 32) | w[1]));
-  else
-    return __uint_as_float(w[0]);
-}
-
-// DPP move of a double (two 32-bit halves); lanes without a source read 0.
-template <int CTRL, int ROWM = 0xF>
-__device__ __forceinline__ double dpp_f64(double x) {
-  const int lo = __double2loint(x), hi = __double2hiint(x);
-  const int l2 = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROWM, 0xF, false);
-  const int h2 = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROWM, 0xF, false);
-  return __hiloint2double(h2, l2);
-}
-
-__device__ __forceinline__ double readlane_f64(double x, int lane) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), lane);
-  return __hiloint2double(hi, lo);
-}
-
-// Maximum of non-negative doubles over the 64 lanes: quad swaps, half-row and
-// row mirrors, then row broadcasts 15 and 31 leave the maximum in lane 63.
-__device__ __forceinline__ double wave_max_nonneg(double v) {
-  v = fmax(v, dpp_f64<0xB1>(v));        // quad_perm [1,0,3,2]
-  v = fmax(v, dpp_f64<0x4E>(v));        // quad_perm [2,3,0,1]
-  v = fmax(v, dpp_f64<0x141>(v));       // row_half_mirror
-  v = fmax(v, dpp_f64<0x140>(v));       // row_mirror
-  v = fmax(v, dpp_f64<0x142, 0xA>(v));  // row_bcast:15 -> rows 1, 3
-  v = fmax(v, dpp_f64<0x143, 0xC>(v));  // row_bcast:31 -> rows 2, 3
-  return readlane_f64(v, 63);
-}
-
-// iamax over one wave: each lane holds (value, row) with value > 0 for a
-// candidate and (0, INT_MAX) otherwise.  Returns the largest value and, among
-// the lanes holding it, the lowest row (uniform across the wave).
-__device__ __forceinline__ void wave_argmax(double v, int row, double &bv, int &br) {
-  const double vmax = wave_max_nonneg(v);
-  const unsigned long long m = __ballot(row != INT_MAX && v == vmax);
-  int best = INT_MAX;
-  unsigned long long mm = m;
-  while (mm) {
-    const int l = __ffsll((long long)mm) - 1;
-    mm &= mm - 1;
-    const int rl = __builtin_amdgcn_readlane(row, l);
-    best = rl < best ? rl : best;
-  }
-  bv = m ? vmax : 0.0;
-  br = best;
-}
 
 // Factors columns [col0, col0+wcols) of rows [row0, m) in place.  Workgroup g
 // owns rows [row0 + g*R, row0 + (g+1)*R) with R = NT*RPT; thread t keeps rows
@@ -236,6 +159,10 @@ struct PanelLeaf {
       int spins = 0;
       bool bail = false;
       for (;;) {
+        if (spins >= pa.wait_limit) {
+          bail = true;
+          break;
+        }
         bool ok = true;
         for (int idx = lane; idx < total; idx += 64) {
           const int q = idx / L, w = idx % L;
@@ -249,7 +176,7 @@ struct PanelLeaf {
           s_rec[total + lane] = (uint32_t)x;
         }
         if (__all(ok)) break;
-        if ((++spins & 255) == 0 && (spins > kSpinLimit || ld_agent(pa.err) != 0)) {
+        if ((++spins & 255) == 0 && ld_agent(pa.err) != 0) {
           bail = true;
           break;
         }
@@ -272,7 +199,7 @@ struct PanelLeaf {
       if (lane == 0) {
         s_win[par] = wr;
         *s_bail = bail ? 1 : 0;
-        if (bail) st_agent(pa.err, 1);
+        if (bail) atomic_or_agent(pa.err, kErrWait);
       }
     }
     __syncthreads();
@@ -337,7 +264,14 @@ struct PanelLeaf {
       win = br;
       prow = s_wrow[par][qw];
     } else {
-      if (!exchange<J>(bv, br, qw)) return false;
+      if (!exchange<J>(bv, br, qw)) {
+        // Given up: leave in-range (identity) pivots for the columns not
+        // decided, so the interchanges queued after this launch stay in
+        // bounds.  Thread 0 of workgroup 0 writes every pivot of the leaf.
+        if (g == 0 && t == 0)
+          for (int jj = J; jj < pa.npiv; ++jj) pa.ipiv[pa.row0 + jj] = pa.row0 + jj;
+        return false;
+      }
       win = s_win[par];
       prow = s_prow[par];
     }
@@ -528,6 +462,7 @@ hipError_t launch_panel(lair_context *ctx, T *a, int64_t lda, int row0, int m, i
   pa.drec = ctx->drec;
   pa.seq = ctx->seq;
   pa.err = ctx->err;
+  pa.wait_limit = ctx->wait_limit;
   pa.stride = ctx->panel_xcd_stride;
   pa.rest1 = rest1;
   pa.left0 = left0;
@@ -633,7 +568,7 @@ template hipError_t launch_laswp<float>(lair_context *, float *, int64_t, int, i
 template <typename T>
 __global__ void __launch_bounds__(64)
     getrs_exact_kernel(int n, int nrhs, const T *lu, int64_t lda, const int64_t *ipiv, T *b,
-                       int64_t ldb) {
+                       int64_t ldb, int32_t *status) {
 #pragma clang fp contract(off)
   extern __shared__ unsigned char smem_raw[];
   T *xs = reinterpret_cast<T *>(smem_raw);  // [n][64]
@@ -642,7 +577,12 @@ __global__ void __launch_bounds__(64)
   if (c >= nrhs) return;
   for (int i = 0; i < n; ++i) xs[i * 64 + t] = b[(int64_t)i * ldb + c];
   for (int i = 0; i < n; ++i) {
-    const int p = (int)ipiv[i];
+    const int64_t p64 = ipiv[i];
+    if (p64 < 0 || p64 >= n) {  // outside [0, n): skipped and reported
+      if (t == 0) atomic_or_agent(status, kErrPivot);
+      continue;
+    }
+    const int p = (int)p64;
     if (p != i) {
       const T tmp = xs[i * 64 + t];
       xs[i * 64 + t] = xs[p * 64 + t];
@@ -675,7 +615,8 @@ hipError_t launch_getrs_exact(lair_context *ctx, int n, int nrhs, const T *lu, i
   if (n > kExactOrderMax) return hipErrorInvalidValue;
   const size_t shm = (size_t)n * 64 * sizeof(T);
   const int slot = prof_begin(ctx, LAIR_KCLASS_GETRS);
-  getrs_exact_kernel<T><<<(nrhs + 63) / 64, 64, shm, ctx->stream>>>(n, nrhs, lu, lda, ipiv, b, ldb);
+  getrs_exact_kernel<T><<<(nrhs + 63) / 64, 64, shm, ctx->stream>>>(n, nrhs, lu, lda, ipiv, b, ldb,
+                                                                     ctx->err);
   prof_end(ctx, slot, LAIR_KCLASS_GETRS, 2.0 * n * n * nrhs);
   return hipGetLastError();
 }
@@ -1262,6 +1203,7 @@ struct CaluArgs {
   unsigned long long *drec;  // [kCaluDiag] tagged words
   uint32_t tag;
   int32_t *err;
+  int wait_limit;  // polls before a wait gives up (0: every wait gives up at once)
   int rest1, left0;
 };
 
@@ -1374,12 +1316,14 @@ __device__ __forceinline__ void wave_gepp(float (&v)[K][kCaluW], const int (&ids
 constexpr int kGatherMax = 9;
 static_assert(kCaluMaxWG * kCaluRec <= kGatherMax * 512, "gather capacity");
 __device__ __forceinline__ bool gather_tagged(const unsigned long long *src, int total, uint32_t tag,
-                                              uint32_t *dst, int t, int nt, const int32_t *err) {
+                                              uint32_t *dst, int t, int nt, const int32_t *err,
+                                              int limit) {
   bool got[kGatherMax];
 #pragma unroll
   for (int q = 0; q < kGatherMax; ++q) got[q] = t + q * nt >= total;
   int spins = 0;
   for (;;) {
+    if (spins >= limit) return false;
     bool all = true;
 #pragma unroll
     for (int q = 0; q < kGatherMax; ++q) {
@@ -1395,7 +1339,7 @@ __device__ __forceinline__ bool gather_tagged(const unsigned long long *src, int
       }
     }
     if (all) return true;
-    if ((++spins & 255) == 0 && (spins > kSpinLimit || ld_agent(err) != 0)) return false;
+    if ((++spins & 255) == 0 && ld_agent(err) != 0) return false;
   }
 }
 
@@ -1491,16 +1435,22 @@ __global__ void __launch_bounds__(512) calu_leaf_kernel(CaluArgs ca) {
     }
     if (t == 0) s_bail = 0;
     __syncthreads();
-    bool ok = gather_tagged(ca.rec, ca.P * kCaluRec, ca.tag, s_fin, t, NT, ca.err);
+    bool ok = gather_tagged(ca.rec, ca.P * kCaluRec, ca.tag, s_fin, t, NT, ca.err, ca.wait_limit);
     if (ok && g != 0)
       ok = gather_tagged(ca.drec, kCaluDiag, ca.tag, reinterpret_cast<uint32_t *>(&s_diag[0][0]), t, NT,
-                         ca.err);
+                         ca.err, ca.wait_limit);
     if (!ok) {
       s_bail = 1;
-      st_agent(ca.err, 1);
+      atomic_or_agent(ca.err, kErrWait);
     }
     __syncthreads();
-    if (s_bail) return;
+    if (s_bail) {
+      // Given up: identity pivots keep the interchanges queued after this
+      // launch in bounds.
+      if (g == 0 && t == 0)
+        for (int i = 0; i < ca.npiv; ++i) ca.ipiv[ca.col0 + i] = ca.col0 + i;
+      return;
+    }
     // Final round on the P x 16 finalists.
     if (wave == 0) {
       const int nf = ca.P * W;
@@ -1668,6 +1618,7 @@ hipError_t launch_panel_calu(lair_context *ctx, float *a, int64_t lda, int c, in
   ca.drec = ctx->crec + (size_t)kCaluMaxWG * kCaluRec;
   ca.tag = ctx->seq + 1u;
   ca.err = ctx->err;
+  ca.wait_limit = ctx->wait_limit;
   ca.rest1 = rest1;
   ca.left0 = left0;
   constexpr int K = 4, R = 512 * K;

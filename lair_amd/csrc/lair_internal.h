@@ -36,6 +36,9 @@ constexpr int kReservedCUs = 32;
 constexpr int kMaxSwapRows = 512;
 // Largest n whose row permutation is composed in LDS.
 constexpr int kPermLdsMax = 32768;
+// Tagged words per workgroup slot of the resident panel's row exchange
+// (>= leaf width * 32-bit words per element).
+constexpr int kRpRowWords = 128;
 
 // Diagonal block size of the dataflow triangular solve.
 template <typename T>
@@ -86,6 +89,14 @@ struct lair_context {
   uint32_t seq = 0;            // tag base of the next panel launch
   int panel_xcd_stride = 1;    // launched slots per panel workgroup (see launch_panel)
   int32_t *err = nullptr;      // set when a bounded wait expires
+  // Polls a cooperative kernel's bounded wait makes before it gives up and
+  // sets *err (0: every wait gives up at once).
+  int32_t wait_limit = 1 << 23;
+  // Resident panel (rpanel.hip): header and row slots, tag base, grid cap.
+  unsigned long long *rp_hdr = nullptr;   // [2][kMaxPanelWG][2]
+  unsigned long long *rp_rowd = nullptr;  // [2][kMaxPanelWG][kRpRowWords]
+  uint32_t rp_seq = 1;
+  int rp_max_wg = lair::kMaxPanelWG;
   int64_t *sing = nullptr;     // last zero-pivot column (device)
   double *norms = nullptr;     // scratch for norm reductions (uint64-ordered doubles)
 
@@ -155,6 +166,15 @@ hipError_t ensure(Buffer &b, size_t bytes);
 template <typename T>
 hipError_t launch_panel(lair_context *ctx, T *a, int64_t lda, int row0, int m, int col0, int wcols,
                         int64_t *ipiv, int rest1, int left0);
+// Resident-panel leaf (rpanel.hip): exact partial pivoting of columns
+// [col0, col0 + wcols) over rows [row0, m), wcols <= nbw, rows kept in LDS.
+template <typename T>
+hipError_t launch_rpanel(lair_context *ctx, T *a, int64_t lda, int row0, int m, int col0, int wcols,
+                         int nbw, int64_t *ipiv);
+// Widest leaf width <= want (128, 64, 32 or 16) the resident panel supports
+// for `rows` rows, or 0.
+template <typename T>
+int rpanel_leaf_width(lair_context *ctx, int want, int rows);
 // Tournament-pivoting FP32 leaf (mixed-precision factorization only).
 hipError_t launch_panel_calu(lair_context *ctx, float *a, int64_t lda, int c, int m, int wcols,
                              int64_t *ipiv, int rest1, int left0);

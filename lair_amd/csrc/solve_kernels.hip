@@ -17,6 +17,7 @@
 //    block (the substitutions of src/lapack/getrs.rs:24-36, blocked).
 #include <climits>
 
+#include "device_util.h"
 #include "lair_internal.h"
 
 namespace lair {
@@ -30,7 +31,6 @@ __device__ __forceinline__ T ld_sc1(const T *p) {
   return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-constexpr int kSolveSpinLimit = 1 << 24;
 
 // Allows a kernel to use up to the full 160 KB of LDS per workgroup.
 // The attribute call's status is consumed here so that it cannot surface as
@@ -164,19 +164,26 @@ template hipError_t launch_rowswap<float>(lair_context *, float *, int64_t, int,
 // perm from the full pivot vector (for factors supplied by the caller):
 // one thread applies the n interchanges to an LDS index array.  When the
 // pivots fit beside it (n <= kPermPivLdsMax) they are staged in LDS first, so
-// the serial loop touches LDS only.  A pivot outside [0, n) is skipped rather
-// than followed out of bounds.
+// the serial loop touches LDS only.  A pivot outside [0, n) (compared as the
+// full int64) is skipped rather than followed out of bounds and reported
+// through the status word (kErrPivot).
 constexpr int kPermPivLdsMax = 20480;  // 2 n ints in 160 KB
-__global__ void __launch_bounds__(256) perm_from_ipiv_kernel(const int64_t *ipiv, int n, int *perm) {
+__global__ void __launch_bounds__(256) perm_from_ipiv_kernel(const int64_t *ipiv, int n, int *perm,
+                                                             int32_t *status) {
   extern __shared__ int sp[];  // [n] perm when it fits, then [n] pivots when both fit
   const bool lds = n <= kPermLdsMax;
   const bool lds_piv = n <= kPermPivLdsMax;
   int *p = lds ? sp : perm;
   int *piv = sp + n;
+  bool bad = false;
   for (int i = threadIdx.x; i < n; i += 256) {
     p[i] = i;
-    if (lds_piv) piv[i] = (int)ipiv[i];
+    const int64_t q = ipiv[i];
+    const bool ok = q >= 0 && q < n;
+    bad = bad || !ok;
+    if (lds_piv) piv[i] = ok ? (int)q : -1;
   }
+  if (bad) atomic_or_agent(status, kErrPivot);
   __syncthreads();
   if (threadIdx.x == 0) {
     if (lds_piv) {
@@ -192,7 +199,8 @@ __global__ void __launch_bounds__(256) perm_from_ipiv_kernel(const int64_t *ipiv
       }
     } else {
       for (int i = 0; i < n; ++i) {
-        const int q = (int)ipiv[i];
+        const int64_t q64 = ipiv[i];
+        const int q = (q64 >= 0 && q64 < n) ? (int)q64 : -1;
         if (q != i && (unsigned)q < (unsigned)n) {
           const int tmp = p[i];
           p[i] = p[q];
@@ -212,7 +220,7 @@ hipError_t launch_perm_from_ipiv(lair_context *ctx, const int64_t *ipiv, int n, 
                                          : (n <= kPermLdsMax ? (size_t)n * sizeof(int) : 0);
   static bool once = (allow_large_lds(perm_from_ipiv_kernel), true);
   (void)once;
-  perm_from_ipiv_kernel<<<1, 256, shm, ctx->stream>>>(ipiv, n, perm);
+  perm_from_ipiv_kernel<<<1, 256, shm, ctx->stream>>>(ipiv, n, perm, ctx->err);
   return hipGetLastError();
 }
 
@@ -407,6 +415,7 @@ struct TrsvArgs {
   uint32_t epoch;
   int forward, trans, dinv_trans;
   int32_t *err;
+  int wait_limit;  // polls before a wait gives up (0: every wait gives up at once)
 };
 
 // Effective matrix E: forward & !trans -> L (strictly lower part of lu);
@@ -512,11 +521,19 @@ __global__ void __launch_bounds__(256) trsv_dataflow_kernel(TrsvArgs<T> p) {
     stash(stage, p.trans != 0);
     if (d + 1 < nb) issue(d + 1, stage);
     if (t == 0) {
-      const uint32_t *f = p.flags + (size_t)j * p.nchunks + ch;
+      // Acquire: the block's x values, stored before its release below, are
+      // visible once its flag reaches this solve's epoch.
+      uint32_t *f = p.flags + (size_t)j * p.nchunks + ch;
       int spins = 0;
-      while ((int32_t)(ld_sc1(f) - p.epoch) < 0) {
-        if (++spins > kSolveSpinLimit || ld_sc1(p.err) != 0) {
-          st_sc1(p.err, 1);
+      for (;;) {
+        if (spins >= p.wait_limit) {
+          atomic_or_agent(p.err, kErrWait);
+          s_bail = 1;
+          break;
+        }
+        if ((int32_t)(__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) - p.epoch) >= 0)
+          break;
+        if ((++spins & 63) == 0 && ld_sc1(p.err) != 0) {
           s_bail = 1;
           break;
         }
@@ -572,9 +589,14 @@ __global__ void __launch_bounds__(256) trsv_dataflow_kernel(TrsvArgs<T> p) {
     for (int q = 0; q < RC; ++q)
       if (q < ncols) st_sc1(p.x + (int64_t)gr * p.ldx + c0 + q, out[q]);
   }
-  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  // Every thread releases its x stores at agent scope (each wave waits for
+  // its own stores), the barrier orders them before thread 0's flag store,
+  // and that release store publishes them to the dependent blocks.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
-  if (t == 0) st_sc1(p.flags + (size_t)i * p.nchunks + ch, p.epoch);
+  if (t == 0)
+    __hip_atomic_store(p.flags + (size_t)i * p.nchunks + ch, p.epoch, __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <typename T>
@@ -615,6 +637,7 @@ hipError_t launch_trsv_dataflow(lair_context *ctx, const T *lu, int64_t lda, con
   a.trans = trans ? 1 : 0;
   a.dinv_trans = dinv_trans ? 1 : 0;
   a.err = ctx->err;
+  a.wait_limit = ctx->wait_limit;
   const unsigned grid = (unsigned)(nblk * nchunks);
   const size_t shm = (size_t)BS * (BS + 16 / sizeof(T)) * sizeof(T);
   static bool once = (allow_large_lds(trsv_dataflow_kernel<T, BS, 1>),
