@@ -175,16 +175,25 @@ static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a,
     if (e != hipSuccess) return hip_status(e, "fill");
     ctx->seq = 1;
   }
+  // Panels that are a single resident leaf leave their row map in
+  // ctx->pmap_*[parity of the panel index]; the others are composed from the
+  // pivots by rowmap_kernel.
+  bool pmap_ok[2] = {false, false};
+  auto parity = [&](int k) { return (k / (int)nb) & 1; };
   // Factors the panel of columns [k, k + kb) on the context's stream.
   auto factor_panel = [&](int k, int kb) -> hipError_t {
     // Resident leaves (rows kept in LDS for the whole leaf).  A leaf narrower
     // than the panel is followed by its interchanges on the panel's other
     // columns, the U rows right of it (trsm) and the update below (gemm).
     const int lw = tournament ? 0 : rpanel_leaf_width<T>(ctx, kb, m - k);
+    const int pp = parity(k);
+    pmap_ok[pp] = lw >= kb;
     if (lw > 0) {
       for (int c = k; c < k + kb; c += lw) {
         const int w = std::min(lw, k + kb - c);
-        hipError_t e2 = launch_rpanel<T>(ctx, a, lda, c, m, c, w, lw, ipiv);
+        hipError_t e2 = pmap_ok[pp] ? launch_rpanel<T>(ctx, a, lda, c, m, c, w, lw, ipiv, ctx->pmap_rows[pp],
+                                                       ctx->pmap_src[pp], ctx->pmap_count[pp])
+                                    : launch_rpanel<T>(ctx, a, lda, c, m, c, w, lw, ipiv);
         if (e2 != hipSuccess) return e2;
         const int npv = std::min(w, m - c);
         if (npv <= 0) continue;
@@ -231,14 +240,36 @@ static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a,
     }
     return hipSuccess;
   };
+  // Panel [k, k + kb)'s interchanges on columns [c0, c1) (and perm).
+  auto swap_cols = [&](int k, int kb, int c0, int c1, int *pm) -> hipError_t {
+    const int pp = parity(k);
+    if (pmap_ok[pp])
+      return launch_rowswap_map<T>(ctx, a, lda, c0, c1, c0, c0, ctx->pmap_rows[pp], ctx->pmap_src[pp],
+                                   ctx->pmap_count[pp], 2 * std::min(kb, m - k), pm);
+    return launch_rowswap<T>(ctx, a, lda, c0, c1, c0, c0, ipiv, k, k + kb, pm);
+  };
   // Brings columns [c0, c1) up to date with panel [k, k + kb): interchanges,
-  // U12 = L11^-1 A12, A22 -= L21 U12.
+  // U12 = L11^-1 A12 (fused for panels up to kTrsmMaxKb wide), A22 -= L21 U12.
   auto update_cols = [&](int k, int kb, int c0, int c1) -> hipError_t {
     if (c1 <= c0) return hipSuccess;
-    hipError_t e2 = launch_rowswap<T>(ctx, a, lda, c0, c1, c0, c0, ipiv, k, k + kb, nullptr);
-    if (e2 != hipSuccess) return e2;
-    e2 = launch_trsm_llu<T>(ctx, kb, c1 - c0, a + (int64_t)k * lda + k, lda, a + (int64_t)k * lda + c0,
-                            lda);
+    hipError_t e2;
+    if (kb <= kTrsmMaxKb) {
+      const int pp = parity(k);
+      if (pmap_ok[pp]) {
+        e2 = launch_swap_trsm<T>(ctx, a, lda, k, kb, c0, c1, ctx->pmap_rows[pp], ctx->pmap_src[pp],
+                                 ctx->pmap_count[pp], 2 * kb);
+      } else {
+        e2 = launch_rowmap(ctx, ipiv, k, k + kb);
+        if (e2 == hipSuccess)
+          e2 = launch_swap_trsm<T>(ctx, a, lda, k, kb, c0, c1, ctx->swap_rows, ctx->swap_src,
+                                   ctx->swap_count, 2 * kb);
+      }
+    } else {
+      e2 = swap_cols(k, kb, c0, c1, nullptr);
+      if (e2 == hipSuccess)
+        e2 = launch_trsm_llu<T>(ctx, kb, c1 - c0, a + (int64_t)k * lda + k, lda, a + (int64_t)k * lda + c0,
+                                lda);
+    }
     if (e2 != hipSuccess) return e2;
     if (m > k + kb)
       e2 = launch_gemm_minus<T>(ctx, mode, m - (k + kb), c1 - c0, kb, a + (int64_t)(k + kb) * lda + k,
@@ -246,6 +277,12 @@ static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a,
                                 a + (int64_t)(k + kb) * lda + c0, lda);
     return e2;
   };
+  // The bulk updates leave the first (tallest) panel's workgroups their CUs:
+  // a resident panel needs all its workgroups running at once.
+  {
+    const int lw0 = tournament ? 0 : rpanel_leaf_width<T>(ctx, std::min<int>((int)nb, kmax), m);
+    ctx->panel_wgs = lw0 > 0 ? rpanel_grid<T>(ctx, lw0, m) : 0;
+  }
   // Look-ahead of depth one.  Stream 1 (the context's stream) carries the
   // critical path: panel k+1's columns are updated first, then panel k+1 is
   // factored.  Stream 2 meanwhile applies panel k to everything else: the
@@ -265,7 +302,7 @@ static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a,
       SideStream side(ctx);
       LAIR_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_panel, 0));
       // Left columns: interchanges only (and the row permutation).
-      e = launch_rowswap<T>(ctx, a, lda, 0, k, 0, 0, ipiv, k, k + kb, perm);
+      e = swap_cols(k, kb, 0, k, perm);
       if (e != hipSuccess) return hip_status(e, "rowswap");
       e = update_cols(k, kb, nxt1, n);
       if (e != hipSuccess) return hip_status(e, "update");
@@ -280,6 +317,7 @@ static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a,
     // Panel k+1's update of the columns right of it reads what stream 2 wrote.
     LAIR_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_bulk, 0));
   }
+  ctx->panel_wgs = 0;
   if (singular) {
     LAIR_HIP(hipMemcpyAsync(singular, ctx->sing, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
     return sync_status(ctx);
@@ -588,7 +626,7 @@ lair_status lair_context_create(lair_context **out) {
   LAIR_HIP(hipMalloc(&ctx->drec, sizeof(unsigned long long) * 2 * 64));
   LAIR_HIP(hipMalloc(&ctx->crec, sizeof(unsigned long long) * kCaluWords));
   LAIR_HIP(hipMalloc(&ctx->err, sizeof(int32_t)));
-  LAIR_HIP(hipMalloc(&ctx->rp_hdr, sizeof(unsigned long long) * 2 * kMaxPanelWG * 2));
+  LAIR_HIP(hipMalloc(&ctx->rp_hdr, sizeof(unsigned long long) * 2 * kMaxPanelWG * kRpHdrWords));
   LAIR_HIP(hipMalloc(&ctx->rp_rowd, sizeof(unsigned long long) * 2 * kMaxPanelWG * kRpRowWords));
   LAIR_HIP(hipMalloc(&ctx->sing, sizeof(int64_t)));
   LAIR_HIP(hipMalloc(&ctx->swap_rows, sizeof(int) * kMaxSwapRows));
@@ -597,6 +635,11 @@ lair_status lair_context_create(lair_context **out) {
   LAIR_HIP(hipMalloc(&ctx->swap_rows_alt, sizeof(int) * kMaxSwapRows));
   LAIR_HIP(hipMalloc(&ctx->swap_src_alt, sizeof(int) * kMaxSwapRows));
   LAIR_HIP(hipMalloc(&ctx->swap_count_alt, sizeof(int)));
+  for (int q = 0; q < 2; ++q) {
+    LAIR_HIP(hipMalloc(&ctx->pmap_rows[q], sizeof(int) * kMaxSwapRows));
+    LAIR_HIP(hipMalloc(&ctx->pmap_src[q], sizeof(int) * kMaxSwapRows));
+    LAIR_HIP(hipMalloc(&ctx->pmap_count[q], sizeof(int)));
+  }
   // The look-ahead stream carries the bulk trailing updates; their grids are
   // capped (see launch_gemm_minus's max_wg) so that kReservedCUs CUs stay free
   // for the critical path on `stream`.
@@ -612,7 +655,8 @@ lair_status lair_context_create(lair_context **out) {
   LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->rec), 4 * kMaxPanelWG * 64, 0u));
   LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->drec), 4 * 64, 0u));
   LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->crec), 2 * kCaluWords, 0u));
-  LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->rp_hdr), 2 * 2 * kMaxPanelWG * 2, 0u));
+  LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->rp_hdr),
+                           2 * 2 * kMaxPanelWG * kRpHdrWords, 0u));
   LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->rp_rowd),
                            2 * 2 * kMaxPanelWG * kRpRowWords, 0u));
   ctx->seq = 1;
@@ -642,6 +686,11 @@ lair_status lair_context_destroy(lair_context *ctx) {
   hipFree(ctx->swap_rows_alt);
   hipFree(ctx->swap_src_alt);
   hipFree(ctx->swap_count_alt);
+  for (int q = 0; q < 2; ++q) {
+    hipFree(ctx->pmap_rows[q]);
+    hipFree(ctx->pmap_src[q]);
+    hipFree(ctx->pmap_count[q]);
+  }
   if (ctx->stream2) {
     hipStreamSynchronize(ctx->stream2);
     hipStreamDestroy(ctx->stream2);

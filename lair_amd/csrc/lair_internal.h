@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <array>
 #include <cstdint>
 #include <map>
@@ -39,6 +40,12 @@ constexpr int kPermLdsMax = 32768;
 // Tagged words per workgroup slot of the resident panel's row exchange
 // (>= leaf width * 32-bit words per element).
 constexpr int kRpRowWords = 128;
+// Columns of B per workgroup in the triangular solves with L11, and the
+// widest panel those kernels take.
+constexpr int kTrsmCW = 16;
+constexpr int kTrsmMaxKb = 128;
+// Tagged header words per workgroup slot of the resident panel's exchange.
+constexpr int kRpHdrWords = 4;
 
 // Diagonal block size of the dataflow triangular solve.
 template <typename T>
@@ -93,7 +100,7 @@ struct lair_context {
   // sets *err (0: every wait gives up at once).
   int32_t wait_limit = 1 << 23;
   // Resident panel (rpanel.hip): header and row slots, tag base, grid cap.
-  unsigned long long *rp_hdr = nullptr;   // [2][kMaxPanelWG][2]
+  unsigned long long *rp_hdr = nullptr;   // [2][kMaxPanelWG][kRpHdrWords]
   unsigned long long *rp_rowd = nullptr;  // [2][kMaxPanelWG][kRpRowWords]
   uint32_t rp_seq = 1;
   int rp_max_wg = lair::kMaxPanelWG;
@@ -104,6 +111,10 @@ struct lair_context {
   // to the look-ahead stream (swapped in by lair::SideStream).
   int *swap_rows = nullptr, *swap_src = nullptr, *swap_count = nullptr;
   int *swap_rows_alt = nullptr, *swap_src_alt = nullptr, *swap_count_alt = nullptr;
+  // Row maps written by single-leaf panels, by panel parity (read by both
+  // streams while the next panel writes the other one).
+  int *pmap_rows[2] = {nullptr, nullptr}, *pmap_src[2] = {nullptr, nullptr};
+  int *pmap_count[2] = {nullptr, nullptr};
 
   // Look-ahead: the trailing update of panel k runs on stream2 while the
   // critical path (next panel's columns, next panel) continues on stream.
@@ -111,6 +122,7 @@ struct lair_context {
   hipEvent_t ev_panel = nullptr, ev_bulk = nullptr;
   int num_cu = 256;
   int bulk_wg_cap = 0;  // > 0: tile GEMMs launch at most this many workgroups
+  int panel_wgs = 0;    // workgroups of the widest resident panel of the current factorization
 
   // Grow-only workspaces.
   lair::Buffer ws_a32, ws_lu64, ws_ipiv, ws_x32, ws_r, ws_tmp, ws_rhs;
@@ -141,7 +153,7 @@ struct SideStream {
   lair_context *ctx;
   explicit SideStream(lair_context *c) : ctx(c) {
     swap();
-    ctx->bulk_wg_cap = (ctx->num_cu - kReservedCUs) / 8 * 8;
+    ctx->bulk_wg_cap = (ctx->num_cu - std::max(kReservedCUs, ctx->panel_wgs)) / 8 * 8;
   }
   ~SideStream() {
     ctx->bulk_wg_cap = 0;
@@ -168,13 +180,19 @@ hipError_t launch_panel(lair_context *ctx, T *a, int64_t lda, int row0, int m, i
                         int64_t *ipiv, int rest1, int left0);
 // Resident-panel leaf (rpanel.hip): exact partial pivoting of columns
 // [col0, col0 + wcols) over rows [row0, m), wcols <= nbw, rows kept in LDS.
+// When map_rows is given, the leaf also writes the row map of its
+// interchanges (2 * npiv gather entries, see launch_rowswap_map).
 template <typename T>
 hipError_t launch_rpanel(lair_context *ctx, T *a, int64_t lda, int row0, int m, int col0, int wcols,
-                         int nbw, int64_t *ipiv);
+                         int nbw, int64_t *ipiv, int *map_rows = nullptr, int *map_src = nullptr,
+                         int *map_count = nullptr);
 // Widest leaf width <= want (128, 64, 32 or 16) the resident panel supports
 // for `rows` rows, or 0.
 template <typename T>
 int rpanel_leaf_width(lair_context *ctx, int want, int rows);
+// Workgroups of a resident leaf of width nbw over `rows` rows (0: unsupported).
+template <typename T>
+int rpanel_grid(lair_context *ctx, int nbw, int rows);
 // Tournament-pivoting FP32 leaf (mixed-precision factorization only).
 hipError_t launch_panel_calu(lair_context *ctx, float *a, int64_t lda, int c, int m, int wcols,
                              int64_t *ipiv, int rest1, int left0);
@@ -200,6 +218,12 @@ hipError_t launch_gemm_minus(lair_context *ctx, int mode, int M, int N, int K, c
 template <typename T>
 hipError_t launch_rowswap(lair_context *ctx, T *a, int64_t lda, int c0, int c1, int s0, int s1,
                           const int64_t *ipiv, int k0, int k1, int *perm);
+// As launch_rowswap, with the row map given (rows[e] receives the old
+// contents of src[e] for e < *count <= nmax; entries with rows[e] == src[e]
+// do nothing), e.g. the one a resident leaf wrote.
+template <typename T>
+hipError_t launch_rowswap_map(lair_context *ctx, T *a, int64_t lda, int c0, int c1, int s0, int s1,
+                              const int *rows, const int *src, const int *count, int nmax, int *perm);
 hipError_t launch_perm_from_ipiv(lair_context *ctx, const int64_t *ipiv, int n, int *perm);
 hipError_t launch_iota(lair_context *ctx, int *p, int n);
 template <typename S, typename D>
@@ -209,6 +233,15 @@ hipError_t launch_permute_rows(lair_context *ctx, int rows, int cols, const int 
 template <typename T>
 hipError_t launch_trsm_llu(lair_context *ctx, int kb, int ncols, const T *L, int64_t ldl, T *B,
                            int64_t ldb);
+// Interchanges (row map, first kb entries = rows k .. k + kb - 1) and
+// U12 = L11^-1 A12 on columns [c0, c1) of panel [k, k + kb), kb <= 128, in one
+// pass per 16-column strip (the rows moved below the panel are written too).
+template <typename T>
+hipError_t launch_swap_trsm(lair_context *ctx, T *a, int64_t lda, int k, int kb, int c0, int c1,
+                            const int *rows, const int *src, const int *count, int nmax);
+// Composes ipiv[k0..k1) into the context's current row-map buffers
+// (ctx->swap_rows / swap_src / swap_count, 2 * (k1 - k0) entries at most).
+hipError_t launch_rowmap(lair_context *ctx, const int64_t *ipiv, int k0, int k1);
 // Inverses of the diagonal blocks (TrsvBlock<T>::BS) of L (unit) and U.
 template <typename T>
 hipError_t launch_trtri_diag(lair_context *ctx, const T *lu, int64_t lda, int n, T *linv, T *uinv);

@@ -30,8 +30,6 @@
 
 namespace lair {
 
-constexpr int kRpNT = 256;              // threads per workgroup
-constexpr int kRpNW = kRpNT / 64;       // waves per workgroup
 constexpr uint32_t kRpNone = 0x1FFFFu;  // header position meaning "no candidate"
 
 template <typename T>
@@ -41,64 +39,96 @@ struct RpArgs {
   int row0, m, col0, wcols, npiv, P;
   int64_t *ipiv;
   int64_t *sing;
-  unsigned long long *hdr;   // [2][kMaxPanelWG][2] header words, by column parity
+  unsigned long long *hdr;   // [2][kMaxPanelWG][kRpHdrWords] header words, by column parity
   unsigned long long *rowd;  // [2][kMaxPanelWG][kRpRowWords] row words, by column parity
+  int *map_rows, *map_src, *map_count;  // row map of the leaf's interchanges (optional)
   uint32_t seq;              // column j's tag is seq + j + 1
   int32_t *err;
   int wait_limit;            // polls before a wait gives up (0: every wait gives up at once)
 };
 
-// Header word `word` of a candidate (v = |value| >= 0, pos = its position or
-// INT_MAX for none).  Word 0: tag (16 bits) | position (17 bits) | the high
-// 31 bits of v (the sign bit of a non-negative value is zero).  FP64 adds
-// word 1: tag (32 bits) | the low 32 bits of v.
+// A candidate's header: its position, its signed pivot value v and u1, its
+// entry in the next column after every earlier pivot's update (what the
+// other workgroups need to continue before the rest of the row arrives).
+//   word 0: tag (16 bits) | position (17 bits) | high 31 bits of |v|
+//   FP32 word 1: tag (31 bits) | sign of v | bits of u1
+//   FP64 word 1: tag (32 bits) | low 32 bits of |v|
+//   FP64 word 2: tag (31 bits) | sign of v | high 32 bits of u1
+//   FP64 word 3: tag (32 bits) | low 32 bits of u1
 template <typename T>
-__device__ __forceinline__ unsigned long long rp_header(int word, uint32_t tag, double v, int pos) {
-  uint32_t hi, lo = 0;
-  if constexpr (sizeof(T) == 8) {
-    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-    hi = (uint32_t)(b >> 32);
-    lo = (uint32_t)b;
-  } else {
-    hi = __float_as_uint((float)v);
+struct RpHeader {
+  static constexpr int W = sizeof(T) == 8 ? 4 : 2;
+  static __device__ __forceinline__ unsigned long long word(int w, uint32_t tag, T v, T u1, int pos) {
+    const uint32_t p = pos == INT_MAX ? kRpNone : (uint32_t)pos;
+    const uint32_t sgn = v < T(0) ? 1u : 0u;
+    const T av = v < T(0) ? -v : v;
+    if constexpr (sizeof(T) == 8) {
+      const unsigned long long b = (unsigned long long)__double_as_longlong(av);
+      const unsigned long long ub = (unsigned long long)__double_as_longlong(u1);
+      if (w == 0)
+        return ((unsigned long long)(tag & 0xFFFFu) << 48) | ((unsigned long long)p << 31) |
+               ((uint32_t)(b >> 32) & 0x7FFFFFFFu);
+      if (w == 1) return tagged(tag, (uint32_t)b);
+      if (w == 2) return ((unsigned long long)(tag & 0x7FFFFFFFu) << 33) | ((unsigned long long)sgn << 32) |
+                         (uint32_t)(ub >> 32);
+      return tagged(tag, (uint32_t)ub);
+    } else {
+      if (w == 0)
+        return ((unsigned long long)(tag & 0xFFFFu) << 48) | ((unsigned long long)p << 31) |
+               (__float_as_uint(av) & 0x7FFFFFFFu);
+      return ((unsigned long long)(tag & 0x7FFFFFFFu) << 33) | ((unsigned long long)sgn << 32) |
+             __float_as_uint(u1);
+    }
   }
-  if (word == 1) return tagged(tag, lo);
-  const uint32_t p = pos == INT_MAX ? kRpNone : (uint32_t)pos;
-  return ((unsigned long long)(tag & 0xFFFFu) << 48) | ((unsigned long long)p << 31) |
-         (hi & 0x7FFFFFFFu);
-}
+  // Tag check of word w.
+  static __device__ __forceinline__ bool ok(int w, unsigned long long x, uint32_t tag) {
+    if (w == 0) return (uint32_t)(x >> 48) == (tag & 0xFFFFu);
+    if (sizeof(T) == 4 || w == 2) return (uint32_t)(x >> 33) == (tag & 0x7FFFFFFFu);
+    return (uint32_t)(x >> 32) == tag;
+  }
+};
 
-template <typename T, int NB, int R>
-__global__ void __launch_bounds__(kRpNT) rpanel_kernel(RpArgs<T> pa) {
+// Waves: wave 0 communicates (publishes, polls, rebuilds pivot rows) and
+// owns no rows; waves 1..NWK own RPT rows per lane, R = 64 * NWK * RPT rows.
+template <typename T, int NB, int NWK, int RPT>
+__global__ void __launch_bounds__(64 * (NWK + 1)) rpanel_kernel(RpArgs<T> pa) {
 #pragma clang fp contract(off)
-  constexpr int RPT = R / kRpNT;
-  constexpr int LD = NB + 1;
+  constexpr int NT = 64 * (NWK + 1);
+  constexpr int NW = 64 * NWK;  // worker threads
+  constexpr int R = NW * RPT;
+  // Column-major tile with a padded column: a worker's sweep along its row
+  // and a wave's copy of one row (lanes along the columns) are both free of
+  // bank conflicts.
+  constexpr int LDC = R + 1;
   constexpr int WPE = sizeof(T) / 4;
   constexpr int NI = (NB + 63) / 64;
+  constexpr int HW = RpHeader<T>::W;
   extern __shared__ unsigned char smem_raw[];
-  T *s_a = reinterpret_cast<T *>(smem_raw);  // [R][LD] the workgroup's rows
-  __shared__ T s_wrow[2][kRpNW][NB];         // wave winners' rows (columns j-1..)
+  T *s_a = reinterpret_cast<T *>(smem_raw);  // [NB][LDC] the workgroup's rows, by column
+  __shared__ T s_wrow[2][NWK][NB];           // worker waves' best rows (columns j-1..)
   __shared__ T s_u[2][NB];                   // pivot rows
-  __shared__ double s_wv[2][kRpNW];
-  __shared__ int s_wp[2][kRpNW];
+  __shared__ double s_wv[2][NWK];
+  __shared__ int s_wp[2][NWK];
+  __shared__ T s_recip[2], s_u1[2];
   __shared__ int s_at[NB];  // original row now at position row0 + e
-  __shared__ int s_pw[2], s_occ[2], s_none[2];
+  __shared__ int s_pw[2], s_occ[2], s_none[2], s_q[2];
   __shared__ int s_fin[R];  // final positions for the write-back
   __shared__ int s_bail;
 
   const int g = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int tw = t - 64;  // worker index (rows tw + s * NW)
   const int base = pa.row0 + g * R;
   const int ncols = pa.wcols;
 
-  // Load: wave w copies rows w, w + 4, ... (coalesced along the row).
-  for (int i = wave; i < R; i += kRpNW) {
+  // Load: wave w copies rows w, w + NWK + 1, ... (coalesced along the row).
+  for (int i = wave; i < R; i += NWK + 1) {
     const int r = base + i;
-    const bool ok = r < pa.m;
-    const T *src = pa.a + (int64_t)(ok ? r : 0) * pa.lda + pa.col0;
+    const bool okr = r < pa.m;
+    const T *src = pa.a + (int64_t)(okr ? r : 0) * pa.lda + pa.col0;
 #pragma unroll
     for (int q = 0; q < NI; ++q) {
       const int c = lane + 64 * q;
-      if (c < NB) s_a[i * LD + c] = (ok && c < ncols) ? src[c] : T(0);
+      if (c < NB) s_a[c * LDC + i] = (okr && c < ncols) ? src[c] : T(0);
     }
   }
   int cpos[RPT];
@@ -106,58 +136,61 @@ __global__ void __launch_bounds__(kRpNT) rpanel_kernel(RpArgs<T> pa) {
   T lr[RPT];
 #pragma unroll
   for (int s = 0; s < RPT; ++s) {
-    cpos[s] = base + t + s * kRpNT;
-    act[s] = cpos[s] < pa.m;
+    cpos[s] = base + tw + s * NW;
+    act[s] = wave > 0 && cpos[s] < pa.m;
     lr[s] = T(0);
   }
-  for (int e = t; e < NB; e += kRpNT) s_at[e] = pa.row0 + e;
+  for (int e = t; e < NB; e += NT) s_at[e] = pa.row0 + e;
   if (t == 0) s_bail = 0;
   __syncthreads();
 
   bool pend = false;  // the previous pivot's update of columns > j is pending
-  for (int j = 0; j < pa.npiv; ++j) {
+  int j = 0;
+  for (; j < pa.npiv; ++j) {
     const int par = j & 1;
     const int d = pa.row0 + j;
     const uint32_t tg = pa.seq + (uint32_t)j + 1u;
 
-    // 1. Candidates of column j: |a| > 0, largest first, lowest position on ties.
-    double bv = 0.0;
-    int bp = INT_MAX, bs = -1;
+    if (wave > 0) {
+      // W1. Candidates of column j: |a| > 0, largest first, lowest position
+      //     on ties; the wave's best row is copied (from column j - 1, the
+      //     multiplier of the pending update) before that update changes it.
+      double bv = 0.0;
+      int bp = INT_MAX, bs = -1;
 #pragma unroll
-    for (int s = 0; s < RPT; ++s) {
-      if (act[s]) {
-        const double x = fabs((double)s_a[(t + s * kRpNT) * LD + j]);
-        if (x > bv || (x == bv && x > 0.0 && cpos[s] < bp)) {
-          bv = x;
-          bp = cpos[s];
-          bs = s;
+      for (int s = 0; s < RPT; ++s) {
+        if (act[s]) {
+          const double x = fabs((double)s_a[j * LDC + tw + s * NW]);
+          if (x > bv || (x == bv && x > 0.0 && cpos[s] < bp)) {
+            bv = x;
+            bp = cpos[s];
+            bs = s;
+          }
         }
       }
+      double wv;
+      int wp;
+      wave_argmax(bv, bp, wv, wp);
+      if (wp != INT_MAX) {
+        const unsigned long long hit = __ballot(bs >= 0 && bp == wp);
+        const int wl = __ffsll((long long)hit) - 1;
+        const int wsel = __builtin_amdgcn_readlane(bs, wl);
+        const int li = (wave - 1) * 64 + wl + wsel * NW;
+        for (int c = (j > 0 ? j - 1 : 0) + lane; c < NB; c += 64) s_wrow[par][wave - 1][c] = s_a[c * LDC + li];
+      }
+      if (lane == 0) {
+        s_wv[par][wave - 1] = wv;
+        s_wp[par][wave - 1] = wp;
+      }
     }
-    double wv;
-    int wp;
-    wave_argmax(bv, bp, wv, wp);
-    if (wp != INT_MAX) {
-      // The wave copies its winner's row from column j - 1 (the multiplier of
-      // the pending update) before the pending update changes it.
-      const unsigned long long hit = __ballot(bs >= 0 && bp == wp);
-      const int wl = __ffsll((long long)hit) - 1;
-      const int wsel = __builtin_amdgcn_readlane(bs, wl);
-      const int li = wave * 64 + wl + wsel * kRpNT;
-      for (int c = (j > 0 ? j - 1 : 0) + lane; c < NB; c += 64) s_wrow[par][wave][c] = s_a[li * LD + c];
-    }
-    if (lane == 0) {
-      s_wv[par][wave] = wv;
-      s_wp[par][wave] = wp;
-    }
-    __syncthreads();
+    __syncthreads();  // B1
 
-    // 2. Wave 0 publishes the workgroup's candidate: header, then the row.
     if (wave == 0) {
+      // C2. The workgroup's candidate, its header and its row.
       double cv = s_wv[par][0];
       int cp = s_wp[par][0], cw = 0;
 #pragma unroll
-      for (int q = 1; q < kRpNW; ++q) {
+      for (int q = 1; q < NWK; ++q) {
         const double ov = s_wv[par][q];
         const int op = s_wp[par][q];
         if (ov > cv || (ov == cv && op < cp)) {
@@ -166,37 +199,33 @@ __global__ void __launch_bounds__(kRpNT) rpanel_kernel(RpArgs<T> pa) {
           cw = q;
         }
       }
-      unsigned long long *hs = pa.hdr + ((size_t)par * kMaxPanelWG + g) * 2;
-      if (lane < WPE) st_agent(hs + lane, rp_header<T>(lane, tg, cv, cp));
+      T cvs = T(0), cu1 = T(0);
+      if (cp != INT_MAX) {
+        const T *wr = s_wrow[par][cw];
+        cvs = wr[j];
+        if (j + 1 < NB) {
+          cu1 = wr[j + 1];
+          if (pend) {
+            const T e = wr[j - 1] * s_u[par ^ 1][j + 1];
+            cu1 = cu1 - e;
+          }
+        }
+      }
+      unsigned long long *hs = pa.hdr + ((size_t)par * kMaxPanelWG + g) * kRpHdrWords;
+      if (lane < HW) st_agent(hs + lane, RpHeader<T>::word(lane, tg, cvs, cu1, cp));
       if (cp != INT_MAX) {
         unsigned long long *rs = pa.rowd + ((size_t)par * kMaxPanelWG + g) * kRpRowWords;
         for (int idx = (j > 0 ? j - 1 : 0) * WPE + lane; idx < NB * WPE; idx += 64)
           st_agent(rs + idx, tagged(tg, word_of<T>(s_wrow[par][cw][idx / WPE], idx % WPE)));
       }
-    }
 
-    // 3. The pending update of columns > j with the previous pivot row.
-    if (pend) {
-#pragma unroll
-      for (int s = 0; s < RPT; ++s) {
-        if (act[s]) {
-          T *row = s_a + (t + s * kRpNT) * LD;
-          const T l = lr[s];
-          for (int c = j + 1; c < ncols; ++c) {
-            const T e = l * s_u[par ^ 1][c];
-            row[c] = row[c] - e;
-          }
-        }
-      }
-    }
-
-    // 4. Wave 0: all headers, the winner, the winner's row.
-    if (wave == 0) {
+      // C3. All headers; the winner by the iamax rule.
       bool bail = false;
       double hv = 0.0;
       int hp = INT_MAX;
+      T hvs = T(0), hu1 = T(0);
       {
-        const unsigned long long *hb = pa.hdr + (size_t)par * kMaxPanelWG * 2;
+        const unsigned long long *hb = pa.hdr + (size_t)par * kMaxPanelWG * kRpHdrWords;
         int spins = 0;
         for (;;) {
           if (spins >= pa.wait_limit) {
@@ -205,20 +234,25 @@ __global__ void __launch_bounds__(kRpNT) rpanel_kernel(RpArgs<T> pa) {
           }
           bool ok = true;
           if (lane < pa.P) {
-            const unsigned long long w0 = ld_agent(hb + lane * 2);
-            uint32_t lo = 0;
-            ok = (uint32_t)(w0 >> 48) == (tg & 0xFFFFu);
-            if constexpr (WPE == 2) {
-              const unsigned long long w1 = ld_agent(hb + lane * 2 + 1);
-              ok = ok && (uint32_t)(w1 >> 32) == tg;
-              lo = (uint32_t)w1;
+            unsigned long long w[HW];
+#pragma unroll
+            for (int k = 0; k < HW; ++k) {
+              w[k] = ld_agent(hb + lane * kRpHdrWords + k);
+              ok = ok && RpHeader<T>::ok(k, w[k], tg);
             }
-            const uint32_t p = (uint32_t)(w0 >> 31) & 0x1FFFFu;
-            const uint32_t hi = (uint32_t)w0 & 0x7FFFFFFFu;
-            if constexpr (WPE == 2)
-              hv = __hiloint2double((int)hi, (int)lo);
-            else
+            const uint32_t p = (uint32_t)(w[0] >> 31) & 0x1FFFFu;
+            const uint32_t hi = (uint32_t)w[0] & 0x7FFFFFFFu;
+            uint32_t sgn;
+            if constexpr (sizeof(T) == 8) {
+              hv = __hiloint2double((int)hi, (int)(uint32_t)w[1]);
+              sgn = (uint32_t)(w[2] >> 32) & 1u;
+              hu1 = __hiloint2double((int)(uint32_t)w[2], (int)(uint32_t)w[3]);
+            } else {
               hv = (double)__uint_as_float(hi);
+              sgn = (uint32_t)(w[1] >> 32) & 1u;
+              hu1 = __uint_as_float((uint32_t)w[1]);
+            }
+            hvs = sgn ? -(T)hv : (T)hv;
             hp = p == kRpNone ? INT_MAX : (int)p;
             if (hp == INT_MAX) hv = 0.0;
           }
@@ -233,12 +267,84 @@ __global__ void __launch_bounds__(kRpNT) rpanel_kernel(RpArgs<T> pa) {
       int gp;
       wave_argmax(hv, hp, gv, gp);
       const bool none = gp == INT_MAX;
-      const int pw = none ? d : gp;
-      if (!bail && !none) {
+      if (lane == 0) {
+        const int occ = s_at[j];
+        const int pw = none ? d : gp;
+        s_pw[par] = pw;
+        s_occ[par] = occ;
+        s_none[par] = none ? 1 : 0;
+        if (pw != d && pw - pa.row0 < NB) s_at[pw - pa.row0] = occ;
+        if (g == 0 && !bail) {
+          pa.ipiv[d] = pw;
+          if (none) *pa.sing = d;
+        }
+        if (bail) {
+          s_bail = 1;
+          atomic_or_agent(pa.err, kErrWait);
+        }
+      }
+      if (!none) {
         const int q = __ffsll((long long)__ballot(lane < pa.P && hp == gp)) - 1;
-        const unsigned long long *rs = pa.rowd + ((size_t)par * kMaxPanelWG + q) * kRpRowWords;
+        T v, u1;
+        if constexpr (sizeof(T) == 8) {
+          v = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(hvs), q),
+                               __builtin_amdgcn_readlane(__double2loint(hvs), q));
+          u1 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(hu1), q),
+                                __builtin_amdgcn_readlane(__double2loint(hu1), q));
+        } else {
+          v = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hvs), q));
+          u1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hu1), q));
+        }
+        if (lane == 0) {
+          s_recip[par] = T(1) / v;
+          s_u1[par] = u1;
+          s_q[par] = q;
+          s_u[par][j] = v;
+        }
+      }
+    } else if (pend) {
+      // W3. The pending update of columns > j with the previous pivot row,
+      //     while wave 0 exchanges; eight columns in flight at a time.
+      const T *u = s_u[par ^ 1];
+#pragma unroll
+      for (int s = 0; s < RPT; ++s) {
+        if (act[s]) {
+          T *row = s_a + tw + s * NW;
+          const T l = lr[s];
+          int c = j + 1;
+          for (; c + 8 <= ncols; c += 8) {
+            T x[8], y[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+              x[q] = row[(c + q) * LDC];
+              y[q] = u[c + q];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+              const T e = l * y[q];
+              row[(c + q) * LDC] = x[q] - e;
+            }
+          }
+          for (; c < ncols; ++c) {
+            const T e = l * u[c];
+            row[c * LDC] = row[c * LDC] - e;
+          }
+        }
+      }
+    }
+    __syncthreads();  // B2
+    if (s_bail) break;
+    const bool none = s_none[par] != 0;
+
+    if (wave == 0) {
+      // C5. The winner's full row, rebuilt as its owner holds it after the
+      //     pending update (same operands, same order), for the next
+      //     iteration's update; read while the workers go on.
+      if (!none) {
+        const unsigned long long *rs = pa.rowd + ((size_t)par * kMaxPanelWG + s_q[par]) * kRpRowWords;
         T rv[NI];
         T mult = T(0);
+        bool bail = false;
         int spins = 0;
         for (;;) {
           if (spins >= pa.wait_limit) {
@@ -248,7 +354,7 @@ __global__ void __launch_bounds__(kRpNT) rpanel_kernel(RpArgs<T> pa) {
           bool ok = true;
 #pragma unroll
           for (int i = 0; i < NI; ++i) {
-            const int c = j + lane + 64 * i;
+            const int c = j + 1 + lane + 64 * i;
             rv[i] = T(0);
             if (c < NB) {
               uint32_t w[WPE];
@@ -277,133 +383,149 @@ __global__ void __launch_bounds__(kRpNT) rpanel_kernel(RpArgs<T> pa) {
             break;
           }
         }
-        // The winner's row as its owner holds it after the pending update.
+        if (bail) {
+          // The status word is set, so the next exchange gives up at once
+          // and the loop ends at the next barrier.
+          if (lane == 0) {
+            s_bail = 1;
+            atomic_or_agent(pa.err, kErrWait);
+          }
+        } else {
 #pragma unroll
-        for (int i = 0; i < NI; ++i) {
-          const int c = j + lane + 64 * i;
-          if (c < NB) {
-            T u = rv[i];
-            if (pend && c > j) {
-              const T e = mult * s_u[par ^ 1][c];
-              u = u - e;
+          for (int i = 0; i < NI; ++i) {
+            const int c = j + 1 + lane + 64 * i;
+            if (c < NB) {
+              T u = rv[i];
+              if (pend) {
+                const T e = mult * s_u[par ^ 1][c];
+                u = u - e;
+              }
+              s_u[par][c] = u;
             }
-            s_u[par][c] = u;
           }
         }
       }
-      if (lane == 0) {
-        const int occ = s_at[j];
-        s_pw[par] = pw;
-        s_occ[par] = occ;
-        s_none[par] = none ? 1 : 0;
-        if (pw != d && pw - pa.row0 < NB) s_at[pw - pa.row0] = occ;
-        if (g == 0 && !bail) {
-          pa.ipiv[d] = pw;
-          if (none) *pa.sing = d;
-        }
-        if (bail) {
-          s_bail = 1;
-          atomic_or_agent(pa.err, kErrWait);
-        }
-      }
-    }
-    __syncthreads();
-    if (s_bail) {
-      // Given up: identity pivots for the columns not decided keep the
-      // interchanges queued after this launch in bounds (thread 0 of
-      // workgroup 0 writes every pivot of the leaf).
-      if (g == 0 && t == 0)
-        for (int jj = j; jj < pa.npiv; ++jj) pa.ipiv[pa.row0 + jj] = pa.row0 + jj;
-      return;
-    }
-
-    // 5. Interchange bookkeeping: the pivot row takes position d, the row at
-    //    position d takes the pivot's old position.
-    const int pw = s_pw[par], occ = s_occ[par];
-    const bool none = s_none[par] != 0;
-#pragma unroll
-    for (int s = 0; s < RPT; ++s) {
-      if (act[s] && cpos[s] == pw) {
-        act[s] = false;
-        cpos[s] = d;
-      } else if (base + t + s * kRpNT == occ) {
-        cpos[s] = pw;
-      }
-    }
-
-    // 6. Multipliers of column j and the update of column j + 1.  A column
-    //    without a nonzero candidate is left unscaled (getrf.rs:72-73).
-    pend = !none;
-    if (!none) {
-      const T recip = T(1) / s_u[par][j];
-      const T u1 = j + 1 < NB ? s_u[par][j + 1] : T(0);
+    } else {
+      // W5. Interchange bookkeeping (the pivot row takes position d, the row
+      //     at d takes the pivot's old position), then the multipliers of
+      //     column j and the update of column j + 1.  A column without a
+      //     nonzero candidate is left unscaled (getrf.rs:72-73).
+      const int pw = s_pw[par], occ = s_occ[par];
 #pragma unroll
       for (int s = 0; s < RPT; ++s) {
-        if (act[s]) {
-          T *row = s_a + (t + s * kRpNT) * LD;
-          const T l = row[j] * recip;
-          row[j] = l;
-          lr[s] = l;
-          if (j + 1 < ncols) {
-            const T e = l * u1;
-            row[j + 1] = row[j + 1] - e;
+        if (act[s] && cpos[s] == pw) {
+          act[s] = false;
+          cpos[s] = d;
+        } else if (base + tw + s * NW == occ) {
+          cpos[s] = pw;
+        }
+      }
+      if (!none) {
+        const T recip = s_recip[par], u1 = s_u1[par];
+#pragma unroll
+        for (int s = 0; s < RPT; ++s) {
+          if (act[s]) {
+            T *row = s_a + tw + s * NW;
+            const T l = row[j * LDC] * recip;
+            row[j * LDC] = l;
+            lr[s] = l;
+            if (j + 1 < ncols) {
+              const T e = l * u1;
+              row[(j + 1) * LDC] = row[(j + 1) * LDC] - e;
+            }
           }
         }
       }
     }
+    pend = !none;
+  }
+  __syncthreads();
+  if (s_bail) {
+    // Given up: identity pivots for the columns not decided keep the
+    // interchanges queued after this launch in bounds (thread 0 of workgroup
+    // 0 writes every pivot of the leaf).
+    if (g == 0 && t == 0) {
+      for (int jj = j; jj < pa.npiv; ++jj) pa.ipiv[pa.row0 + jj] = pa.row0 + jj;
+      if (pa.map_count) *pa.map_count = 0;
+    }
+    return;
   }
 
   // The last pivot's update of the columns beyond the pivots (wide leaves).
-  if (pend) {
+  if (pend && wave > 0) {
     const int jl = pa.npiv - 1;
 #pragma unroll
     for (int s = 0; s < RPT; ++s) {
       if (act[s]) {
-        T *row = s_a + (t + s * kRpNT) * LD;
+        T *row = s_a + tw + s * NW;
         for (int c = jl + 2; c < ncols; ++c) {
           const T e = lr[s] * s_u[jl & 1][c];
-          row[c] = row[c] - e;
+          row[c * LDC] = row[c * LDC] - e;
         }
       }
     }
   }
 
-  // Write-back: every row to its final position.
+  // Row map of the leaf's interchanges in gather form (rows[e] receives the
+  // old contents of src[e]; rows[e] == src[e] is a no-op).  Slot e < npiv:
+  // the final occupant of position row0 + e.  Slot npiv + e: where the row
+  // that started at row0 + e went, when it left [row0, row0 + npiv).  Each
+  // slot is written once, by the owner of the row it describes.
+  if (wave > 0) {
 #pragma unroll
-  for (int s = 0; s < RPT; ++s) s_fin[t + s * kRpNT] = base + t + s * kRpNT < pa.m ? cpos[s] : -1;
+    for (int s = 0; s < RPT; ++s) {
+      const int orig = base + tw + s * NW;
+      const bool valid = orig < pa.m;
+      s_fin[tw + s * NW] = valid ? cpos[s] : -1;
+      if (pa.map_rows && valid) {
+        const int top1 = pa.row0 + pa.npiv;
+        const int fin = cpos[s];
+        if (fin < top1) {
+          pa.map_rows[fin - pa.row0] = fin;
+          pa.map_src[fin - pa.row0] = orig;
+        }
+        if (orig < top1) {
+          pa.map_rows[pa.npiv + orig - pa.row0] = fin < top1 ? orig : fin;
+          pa.map_src[pa.npiv + orig - pa.row0] = orig;
+        }
+      }
+    }
+  }
+  if (pa.map_count && g == 0 && t == 0) *pa.map_count = 2 * pa.npiv;
   __syncthreads();
-  for (int i = wave; i < R; i += kRpNW) {
+
+  // Write-back: every row to its final position.
+  for (int i = wave; i < R; i += NWK + 1) {
     const int dst = s_fin[i];
     if (dst < 0) continue;
     T *out = pa.a + (int64_t)dst * pa.lda + pa.col0;
 #pragma unroll
     for (int q = 0; q < NI; ++q) {
       const int c = lane + 64 * q;
-      if (c < ncols) out[c] = s_a[i * LD + c];
+      if (c < ncols) out[c] = s_a[c * LDC + i];
     }
   }
 }
 
-template <typename T, int NB, int R>
+template <typename T, int NB, int NWK, int RPT>
 static hipError_t rp_launch(lair_context *ctx, const RpArgs<T> &pa) {
-  auto *k = &rpanel_kernel<T, NB, R>;
-  const size_t shm = (size_t)R * (NB + 1) * sizeof(T);
+  auto *k = &rpanel_kernel<T, NB, NWK, RPT>;
+  constexpr int R = 64 * NWK * RPT;
+  const size_t shm = (size_t)NB * (R + 1) * sizeof(T);
   static bool once = ((void)hipFuncSetAttribute(reinterpret_cast<const void *>(k),
                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                                 (int)shm),
                       (void)hipGetLastError(), true);
   (void)once;
-  k<<<pa.P, kRpNT, shm, ctx->stream>>>(pa);
+  k<<<pa.P, 64 * (NWK + 1), shm, ctx->stream>>>(pa);
   return hipGetLastError();
 }
 
-// Rows per workgroup for a leaf of width nbw and `rows` rows: the smallest R
-// with an instance whose grid fits ctx->rp_max_wg; 0 if none.
 template <typename T>
 int rpanel_rows_per_wg(lair_context *ctx, int nbw, int rows) {
   const int maxp = std::min(ctx->rp_max_wg, kMaxPanelWG);
-  static const int kF[][2] = {{128, 256}, {64, 256}, {64, 512}, {32, 512}, {32, 1024}};
-  static const int kD[][2] = {{64, 256}, {32, 256}, {32, 512}, {16, 512}, {16, 1024}};
+  static const int kF[][2] = {{128, 256}, {64, 512}, {32, 1024}};
+  static const int kD[][2] = {{64, 256}, {32, 512}, {16, 1024}};
   const auto &tab = sizeof(T) == 4 ? kF : kD;
   for (const auto &e : tab)
     if (e[0] == nbw && (rows + e[1] - 1) / e[1] <= maxp) return e[1];
@@ -420,7 +542,7 @@ int rpanel_leaf_width(lair_context *ctx, int want, int rows) {
 
 template <typename T>
 hipError_t launch_rpanel(lair_context *ctx, T *a, int64_t lda, int row0, int m, int col0, int wcols,
-                         int nbw, int64_t *ipiv) {
+                         int nbw, int64_t *ipiv, int *map_rows, int *map_src, int *map_count) {
   const int rows = m - row0;
   if (rows <= 0 || wcols <= 0) return hipSuccess;
   if (wcols > nbw || m >= (int)kRpNone) return hipErrorInvalidValue;
@@ -430,7 +552,7 @@ hipError_t launch_rpanel(lair_context *ctx, T *a, int64_t lda, int row0, int m, 
   // they could wrap inside this launch.
   if ((ctx->rp_seq & 0xFFFFu) + (uint32_t)nbw + 2u > 0xFFFFu || ctx->rp_seq > 0xF0000000u) {
     hipError_t e = launch_fill_u32(ctx, reinterpret_cast<uint32_t *>(ctx->rp_hdr),
-                                   2 * 2 * kMaxPanelWG * 2, 0u);
+                                   2 * 2 * kMaxPanelWG * kRpHdrWords, 0u);
     if (e == hipSuccess)
       e = launch_fill_u32(ctx, reinterpret_cast<uint32_t *>(ctx->rp_rowd),
                           2 * 2 * kMaxPanelWG * kRpRowWords, 0u);
@@ -448,6 +570,9 @@ hipError_t launch_rpanel(lair_context *ctx, T *a, int64_t lda, int row0, int m, 
   pa.P = (rows + R - 1) / R;
   pa.ipiv = ipiv;
   pa.sing = ctx->sing;
+  pa.map_rows = map_rows;
+  pa.map_src = map_src;
+  pa.map_count = map_count;
   pa.hdr = ctx->rp_hdr;
   pa.rowd = ctx->rp_rowd;
   pa.seq = ctx->rp_seq;
@@ -456,17 +581,13 @@ hipError_t launch_rpanel(lair_context *ctx, T *a, int64_t lda, int row0, int m, 
   const int slot = prof_begin(ctx, LAIR_KCLASS_PANEL);
   hipError_t e;
   if constexpr (sizeof(T) == 4) {
-    if (nbw == 128) e = rp_launch<T, 128, 256>(ctx, pa);
-    else if (nbw == 64 && R == 256) e = rp_launch<T, 64, 256>(ctx, pa);
-    else if (nbw == 64) e = rp_launch<T, 64, 512>(ctx, pa);
-    else if (R == 512) e = rp_launch<T, 32, 512>(ctx, pa);
-    else e = rp_launch<T, 32, 1024>(ctx, pa);
+    if (nbw == 128) e = rp_launch<T, 128, 4, 1>(ctx, pa);
+    else if (nbw == 64) e = rp_launch<T, 64, 4, 2>(ctx, pa);
+    else e = rp_launch<T, 32, 4, 4>(ctx, pa);
   } else {
-    if (nbw == 64) e = rp_launch<T, 64, 256>(ctx, pa);
-    else if (nbw == 32 && R == 256) e = rp_launch<T, 32, 256>(ctx, pa);
-    else if (nbw == 32) e = rp_launch<T, 32, 512>(ctx, pa);
-    else if (R == 512) e = rp_launch<T, 16, 512>(ctx, pa);
-    else e = rp_launch<T, 16, 1024>(ctx, pa);
+    if (nbw == 64) e = rp_launch<T, 64, 4, 1>(ctx, pa);
+    else if (nbw == 32) e = rp_launch<T, 32, 4, 2>(ctx, pa);
+    else e = rp_launch<T, 16, 4, 4>(ctx, pa);
   }
   prof_end(ctx, slot, LAIR_KCLASS_PANEL, (double)pa.npiv * rows * wcols,
            2.0 * sizeof(T) * (double)rows * wcols);
@@ -474,11 +595,19 @@ hipError_t launch_rpanel(lair_context *ctx, T *a, int64_t lda, int row0, int m, 
   return e;
 }
 
+template <typename T>
+int rpanel_grid(lair_context *ctx, int nbw, int rows) {
+  const int R = rpanel_rows_per_wg<T>(ctx, nbw, rows);
+  return R > 0 ? (rows + R - 1) / R : 0;
+}
+
+template int rpanel_grid<float>(lair_context *, int, int);
+template int rpanel_grid<double>(lair_context *, int, int);
 template int rpanel_leaf_width<float>(lair_context *, int, int);
 template int rpanel_leaf_width<double>(lair_context *, int, int);
 template hipError_t launch_rpanel<float>(lair_context *, float *, int64_t, int, int, int, int, int,
-                                         int64_t *);
+                                         int64_t *, int *, int *, int *);
 template hipError_t launch_rpanel<double>(lair_context *, double *, int64_t, int, int, int, int, int,
-                                          int64_t *);
+                                          int64_t *, int *, int *, int *);
 
 }  // namespace lair

@@ -110,9 +110,22 @@ __global__ void __launch_bounds__(256)
   const bool act = i < total;
   const int c = act ? ((i < n1) ? c0 + i : s1 + (i - n1)) : 0;
   const int nr = *count;
-  for (int e = rg; e < nr; e += 8) {
-    const int s = src[e];
-    if (act && s != rows[e]) buf[e * 32 + cl] = a[(int64_t)s * lda + c];
+  // Batches of 8 independent loads per thread.
+  for (int e0 = rg; e0 < nr; e0 += 64) {
+    T v[8];
+    int sr[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = e0 + 8 * q;
+      sr[q] = e < nr ? src[e] : -1;
+      const bool mv = sr[q] >= 0 && sr[q] != rows[e];
+      v[q] = (act && mv) ? a[(int64_t)sr[q] * lda + c] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = e0 + 8 * q;
+      if (e < nr) buf[e * 32 + cl] = v[q];
+    }
   }
   __syncthreads();
   for (int e = rg; e < nr; e += 8) {
@@ -125,9 +138,33 @@ __global__ void __launch_bounds__(256)
     permgather_kernel(int *perm, const int *rows, const int *src, const int *count) {
   __shared__ int tmp[kMaxSwapRows];
   const int nr = *count;
-  for (int e = threadIdx.x; e < nr; e += 256) tmp[e] = perm[src[e]];
+  // Entries with rows[e] == src[e] do nothing (and may share their row with
+  // an entry that moves one).
+  for (int e = threadIdx.x; e < nr; e += 256)
+    if (src[e] != rows[e]) tmp[e] = perm[src[e]];
   __syncthreads();
-  for (int e = threadIdx.x; e < nr; e += 256) perm[rows[e]] = tmp[e];
+  for (int e = threadIdx.x; e < nr; e += 256)
+    if (src[e] != rows[e]) perm[rows[e]] = tmp[e];
+}
+
+// The gather of a prepared row map over columns [c0, c1) minus [s0, s1).
+template <typename T>
+static hipError_t rowgather(lair_context *ctx, T *a, int64_t lda, int c0, int c1, int s0, int s1,
+                            const int *rows, const int *src, const int *count, int nmax, int *perm) {
+  if (s0 < c0) s0 = c0;
+  if (s1 > c1) s1 = c1;
+  if (s1 < s0) s1 = s0;
+  const int n1 = s0 - c0;
+  const int total = n1 + (c1 - s1);
+  if (total > 0) {
+    const size_t shm = (size_t)nmax * 32 * sizeof(T);
+    static bool once = (allow_large_lds(rowgather_kernel<T>), true);
+    (void)once;
+    rowgather_kernel<T><<<(total + 31) / 32, 256, shm, ctx->stream>>>(a, lda, rows, src, count, c0, n1, s1,
+                                                                       total);
+  }
+  if (perm) permgather_kernel<<<1, 256, 0, ctx->stream>>>(perm, rows, src, count);
+  return hipGetLastError();
 }
 
 template <typename T>
@@ -135,26 +172,30 @@ hipError_t launch_rowswap(lair_context *ctx, T *a, int64_t lda, int c0, int c1, 
                           const int64_t *ipiv, int k0, int k1, int *perm) {
   if (k1 <= k0) return hipSuccess;
   if (k1 - k0 > kMaxSwapRows / 2) return hipErrorInvalidValue;
-  if (s0 < c0) s0 = c0;
-  if (s1 > c1) s1 = c1;
-  if (s1 < s0) s1 = s0;
-  const int n1 = s0 - c0;
-  const int total = n1 + (c1 - s1);
   const int slot = prof_begin(ctx, LAIR_KCLASS_LASWP);
   rowmap_kernel<<<1, 64, 0, ctx->stream>>>(ipiv, k0, k1, ctx->swap_rows, ctx->swap_src,
                                            ctx->swap_count);
-  if (total > 0) {
-    const size_t shm = (size_t)2 * (k1 - k0) * 32 * sizeof(T);
-    static bool once = (allow_large_lds(rowgather_kernel<T>), true);
-    (void)once;
-    rowgather_kernel<T><<<(total + 31) / 32, 256, shm, ctx->stream>>>(
-        a, lda, ctx->swap_rows, ctx->swap_src, ctx->swap_count, c0, n1, s1, total);
-  }
-  if (perm) permgather_kernel<<<1, 256, 0, ctx->stream>>>(perm, ctx->swap_rows, ctx->swap_src,
-                                                          ctx->swap_count);
-  prof_end(ctx, slot, LAIR_KCLASS_LASWP, 0.0, 4.0 * sizeof(T) * (double)(k1 - k0) * total);
-  return hipGetLastError();
+  hipError_t e = rowgather<T>(ctx, a, lda, c0, c1, s0, s1, ctx->swap_rows, ctx->swap_src, ctx->swap_count,
+                              2 * (k1 - k0), perm);
+  prof_end(ctx, slot, LAIR_KCLASS_LASWP, 0.0, 4.0 * sizeof(T) * (double)(k1 - k0) * (c1 - c0));
+  return e;
 }
+
+template <typename T>
+hipError_t launch_rowswap_map(lair_context *ctx, T *a, int64_t lda, int c0, int c1, int s0, int s1,
+                              const int *rows, const int *src, const int *count, int nmax, int *perm) {
+  if (nmax <= 0) return hipSuccess;
+  if (nmax > kMaxSwapRows) return hipErrorInvalidValue;
+  const int slot = prof_begin(ctx, LAIR_KCLASS_LASWP);
+  hipError_t e = rowgather<T>(ctx, a, lda, c0, c1, s0, s1, rows, src, count, nmax, perm);
+  prof_end(ctx, slot, LAIR_KCLASS_LASWP, 0.0, 2.0 * sizeof(T) * (double)nmax * (c1 - c0));
+  return e;
+}
+
+template hipError_t launch_rowswap_map<double>(lair_context *, double *, int64_t, int, int, int, int,
+                                               const int *, const int *, const int *, int, int *);
+template hipError_t launch_rowswap_map<float>(lair_context *, float *, int64_t, int, int, int, int,
+                                              const int *, const int *, const int *, int, int *);
 
 template hipError_t launch_rowswap<double>(lair_context *, double *, int64_t, int, int, int, int,
                                            const int64_t *, int, int, int *);
@@ -307,10 +348,220 @@ __global__ void __launch_bounds__(256)
     for (int i = rg; i < kb; i += 4) B[(int64_t)i * ldb + c] = xs[i * 64 + col];
 }
 
+// B := L^-1 B for kb <= 128 with 16 columns of B per workgroup, so that a
+// 128-column strip (the next panel's columns) spreads over 8 workgroups and a
+// wide strip over many.  L (strictly lower part) and the workgroup's columns
+// are staged in LDS.  Diagonal 16-row blocks are solved with one thread per
+// column; the rows below each block are then updated by all threads.  Every
+// element receives its subtractions b -= b[k] * L[i][k] one at a time in
+// increasing k (src/blas/trsm.rs:13-21 applies them in the same order).
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    trsm_llu16_kernel(int kb, int ncols, const T *__restrict__ L, int64_t ldl, T *__restrict__ B,
+                      int64_t ldb) {
+#pragma clang fp contract(off)
+  constexpr int CW = kTrsmCW, LX = CW + 1;
+  extern __shared__ unsigned char smem_raw[];
+  const int LL = kb + 1;
+  T *sL = reinterpret_cast<T *>(smem_raw);  // [kb][kb + 1]
+  T *sX = sL + (size_t)kb * LL;             // [kb][CW + 1]
+  const int t = threadIdx.x;
+  const int c0 = blockIdx.x * CW;
+  const int nc = ncols - c0 < CW ? ncols - c0 : CW;
+  for (int e = t; e < kb * kb; e += 256) {
+    const int r = e / kb, c = e % kb;
+    if (c < r) sL[r * LL + c] = L[(int64_t)r * ldl + c];
+  }
+  for (int e = t; e < kb * CW; e += 256) {
+    const int r = e / CW, c = e % CW;
+    sX[r * LX + c] = c < nc ? B[(int64_t)r * ldb + c0 + c] : T(0);
+  }
+  __syncthreads();
+  for (int b0 = 0; b0 < kb; b0 += 16) {
+    const int bs = kb - b0 < 16 ? kb - b0 : 16;
+    if (t < CW) {
+      for (int i = 1; i < bs; ++i) {
+        T acc = sX[(b0 + i) * LX + t];
+        for (int p = 0; p < i; ++p) {
+          const T e = sX[(b0 + p) * LX + t] * sL[(b0 + i) * LL + b0 + p];
+          acc = acc - e;
+        }
+        sX[(b0 + i) * LX + t] = acc;
+      }
+    }
+    __syncthreads();
+    const int rows = kb - b0 - bs;
+    for (int e = t; e < rows * CW; e += 256) {
+      const int r = b0 + bs + e / CW, c = e % CW;
+      T acc = sX[r * LX + c];
+      for (int p = 0; p < bs; ++p) {
+        const T pr = sX[(b0 + p) * LX + c] * sL[r * LL + b0 + p];
+        acc = acc - pr;
+      }
+      sX[r * LX + c] = acc;
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < kb * CW; e += 256) {
+    const int r = e / CW, c = e % CW;
+    if (c < nc) B[(int64_t)r * ldb + c0 + c] = sX[r * LX + c];
+  }
+}
+
+// Interchanges then U12 = L11^-1 A12 for columns [c0, c1) of a panel
+// [k, k + kb) (kb <= 128): the laswp + trsm steps of recursive_inner
+// (src/lapack/getrf.rs:270-278) fused per 16-column strip.  The strip's rows
+// named by the row map (rows[e] receives the old contents of src[e]; the
+// first kb entries are rows k .. k + kb - 1 in order) are read once into LDS;
+// the rows moved below the panel are written back at once, and the top kb
+// rows are solved in LDS and written as U12.  Diagonal 16-row blocks are
+// solved in registers by one thread per column; the rows below each block
+// are then updated by all threads.  Every element receives its subtractions
+// b -= b[p] * L[i][p] one at a time in increasing p (src/blas/trsm.rs:13-21
+// applies them in the same order).
+template <typename T>
+__global__ void __launch_bounds__(256)
+    swap_trsm_kernel(T *__restrict__ a, int64_t lda, int k, int kb, int c0, int ncols,
+                     const int *__restrict__ rows, const int *__restrict__ src,
+                     const int *__restrict__ count, int nmax) {
+#pragma clang fp contract(off)
+  constexpr int CW = kTrsmCW, LX = CW + 1, RG = 256 / CW;
+  extern __shared__ unsigned char smem_raw[];
+  T *sL = reinterpret_cast<T *>(smem_raw);             // strictly lower L11, row r at r(r-1)/2
+  T *sX = sL + ((size_t)kb * (kb - 1) / 2 + 1);         // [nmax][CW + 1] map rows' old values
+  int *s_rows = reinterpret_cast<int *>(sX + (size_t)nmax * LX);  // [nmax]
+  int *s_src = s_rows + nmax;                            // [nmax]
+  const int t = threadIdx.x;
+  const int cs = c0 + blockIdx.x * CW;
+  const int nc = c0 + ncols - cs < CW ? c0 + ncols - cs : CW;
+  const int nr = *count;
+  for (int e = t; e < nr; e += 256) {
+    s_rows[e] = rows[e];
+    s_src[e] = src[e];
+  }
+  // L11 in batches of 16 independent loads per thread (the square is read,
+  // its strictly lower part kept).
+  const T *L = a + (int64_t)k * lda + k;
+  for (int b = 0; b < kb * kb; b += 256 * 16) {
+    T v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int idx = b + q * 256 + t, r = idx / kb, p = idx % kb;
+      v[q] = (idx < kb * kb && p < r) ? L[(int64_t)r * lda + p] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int idx = b + q * 256 + t, r = idx / kb, p = idx % kb;
+      if (idx < kb * kb && p < r) sL[r * (r - 1) / 2 + p] = v[q];
+    }
+  }
+  __syncthreads();
+  const int c = t % CW, rg = t / CW;
+  for (int e0 = rg; e0 < nr; e0 += RG * 16) {
+    T v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int e = e0 + q * RG;
+      v[q] = (e < nr && c < nc) ? a[(int64_t)s_src[e] * lda + cs + c] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int e = e0 + q * RG;
+      if (e < nr) sX[e * LX + c] = v[q];
+    }
+  }
+  __syncthreads();
+  // Rows moved below the panel (entries kb.. that move something).
+  for (int e = kb + rg; e < nr; e += RG)
+    if (c < nc && s_rows[e] != s_src[e]) a[(int64_t)s_rows[e] * lda + cs + c] = sX[e * LX + c];
+  for (int b0 = 0; b0 < kb; b0 += 16) {
+    const int bs = kb - b0 < 16 ? kb - b0 : 16;
+    if (t < CW) {
+      T x[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) x[i] = i < bs ? sX[(b0 + i) * LX + t] : T(0);
+#pragma unroll
+      for (int i = 1; i < 16; ++i) {
+        if (i < bs) {
+          const T *lr = sL + (b0 + i) * (b0 + i - 1) / 2 + b0;
+#pragma unroll
+          for (int p = 0; p < i; ++p) {
+            const T e = x[p] * lr[p];
+            x[i] = x[i] - e;
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 16; ++i)
+        if (i < bs) sX[(b0 + i) * LX + t] = x[i];
+    }
+    __syncthreads();
+    T xb[16];
+#pragma unroll
+    for (int p = 0; p < 16; ++p) xb[p] = p < bs ? sX[(b0 + p) * LX + c] : T(0);
+    for (int r = b0 + bs + rg; r < kb; r += RG) {
+      const T *lr = sL + r * (r - 1) / 2 + b0;
+      T acc = sX[r * LX + c];
+#pragma unroll
+      for (int p = 0; p < 16; ++p) {
+        if (p < bs) {
+          const T pr = xb[p] * lr[p];
+          acc = acc - pr;
+        }
+      }
+      sX[r * LX + c] = acc;
+    }
+    __syncthreads();
+  }
+  for (int r = rg; r < kb; r += RG)
+    if (c < nc) a[(int64_t)(k + r) * lda + cs + c] = sX[r * LX + c];
+}
+
+template <typename T>
+hipError_t launch_swap_trsm(lair_context *ctx, T *a, int64_t lda, int k, int kb, int c0, int c1,
+                            const int *rows, const int *src, const int *count, int nmax) {
+  if (c1 <= c0 || kb <= 0) return hipSuccess;
+  if (kb > kTrsmMaxKb || nmax > kMaxSwapRows || nmax < kb) return hipErrorInvalidValue;
+  const size_t shm = sizeof(T) * ((size_t)kb * (kb - 1) / 2 + 1 + (size_t)nmax * (kTrsmCW + 1)) +
+                     2 * sizeof(int) * (size_t)nmax + 16;
+  static bool once = (allow_large_lds(swap_trsm_kernel<T>), true);
+  (void)once;
+  const int slot = prof_begin(ctx, LAIR_KCLASS_TRSM);
+  swap_trsm_kernel<T><<<(c1 - c0 + kTrsmCW - 1) / kTrsmCW, 256, shm, ctx->stream>>>(a, lda, k, kb, c0, c1 - c0,
+                                                                                      rows, src, count, nmax);
+  prof_end(ctx, slot, LAIR_KCLASS_TRSM, (double)kb * kb * (c1 - c0),
+           sizeof(T) * (2.0 * (kb + nmax) * (c1 - c0) + 0.5 * kb * kb));
+  return hipGetLastError();
+}
+
+template hipError_t launch_swap_trsm<double>(lair_context *, double *, int64_t, int, int, int, int,
+                                             const int *, const int *, const int *, int);
+template hipError_t launch_swap_trsm<float>(lair_context *, float *, int64_t, int, int, int, int,
+                                            const int *, const int *, const int *, int);
+
+hipError_t launch_rowmap(lair_context *ctx, const int64_t *ipiv, int k0, int k1) {
+  if (k1 <= k0) return hipSuccess;
+  if (k1 - k0 > kMaxSwapRows / 2) return hipErrorInvalidValue;
+  rowmap_kernel<<<1, 64, 0, ctx->stream>>>(ipiv, k0, k1, ctx->swap_rows, ctx->swap_src, ctx->swap_count);
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_trsm_llu(lair_context *ctx, int kb, int ncols, const T *L, int64_t ldl, T *B,
                            int64_t ldb) {
   if (kb <= 0 || ncols <= 0) return hipSuccess;
+  if (kb <= kTrsmMaxKb) {
+    const size_t shm = sizeof(T) * ((size_t)kb * (kb + 1) + (size_t)kb * (kTrsmCW + 1));
+    static bool once16 = (allow_large_lds(trsm_llu16_kernel<T>), true);
+    (void)once16;
+    const int slot = prof_begin(ctx, LAIR_KCLASS_TRSM);
+    trsm_llu16_kernel<T><<<(ncols + kTrsmCW - 1) / kTrsmCW, 256, shm, ctx->stream>>>(kb, ncols, L, ldl, B,
+                                                                                      ldb);
+    prof_end(ctx, slot, LAIR_KCLASS_TRSM, (double)kb * kb * ncols,
+             sizeof(T) * (2.0 * kb * ncols + 0.5 * kb * kb));
+    return hipGetLastError();
+  }
   const size_t shm = (size_t)kb * 64 * sizeof(T);
   if (shm > 160 * 1024) return hipErrorInvalidValue;
   static bool once = (allow_large_lds(trsm_llu_kernel<T>), true);
