@@ -9,9 +9,13 @@ Workload (one "step" = one full solve A X = B from device-resident FP64 data):
 
 Printed JSON line: value = milliseconds per mixed-precision solve (lower is
 better) with the FP64 solve time, the speedup and both backward errors in
-`config`; `roofline` for the dominant kernel (trailing-matrix update) from HIP
-events recorded on the library's stream during the timed steps; `cpu_baseline`
-from the CPU oracle on a bounded sample.
+`config` (1 right-hand side; the 64-column case is reported beside it);
+`roofline` for the kernel that dominates the critical path (the resident panel
+leaf) and `roofline_gemm` for the trailing-matrix update beside it, both from
+HIP events recorded on the library's streams during the timed steps;
+`cpu_baseline` from the CPU oracle on a bounded sample.  `config.c1` and
+`config.c2` time the plain FP64 getrf + getrs at n=512 (one right-hand side)
+and n=8192 (16 right-hand sides, autotuned panel width).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 16384] [--nrhs 1]
 """
@@ -180,17 +184,19 @@ def bench_adjoint(ctx, gl, steps, n=4096, nrhs=64):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trailing_update_traffic.json")
 
 
-def measured_traffic(head):
-    """HBM bytes per launch of the headline trailing update, from the committed
-    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by
-    the same counters' reading for a copy kernel of known size); None when no
-    summary for this variant exists."""
+def measured_traffic(key):
+    """HBM bytes per launch of a kernel class ("panel", or the trailing update
+    of "fp32" / "bf16x6"), from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the same counters' reading
+    for a copy kernel of known size); None when no summary for it exists."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    entry = t.get("bf16x6" if head.startswith("bf16x6") else "fp32")
+    if key != "panel":
+        key = "bf16x6" if key.startswith("bf16x6") else "fp32"
+    entry = t.get(key)
     return entry.get("hbm_bytes_per_launch") if entry else None
 
 
@@ -202,23 +208,83 @@ def prof_read(ctx, kname):
     return ms.value, cnt.value, fl.value, by.value
 
 
-def cpu_baseline(n_target, n_sample):
-    """The oracle's restatement of getrf_row_major + getrs (single thread) on an
-    n_sample Gaussian system, scaled to n_target by the n^3 flop ratio."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_solve_ms(n, seed=99):
+    """The oracle's restatement of getrf_row_major + getrs on one core for an
+    n x n N(0,1) system (milliseconds)."""
     from lair_amd import inputs
     from oracle import oracle as O
-    a = inputs.gaussian_matrix(99, n_sample, n_sample)
-    b = inputs.gaussian_matrix(100, n_sample, 1)
+    a = inputs.gaussian_matrix(seed, n, n)
+    b = inputs.gaussian_matrix(seed + 1, n, 1)
     t = time.perf_counter()
     piv, _ = O.getrf(a, nthreads=1)
     O.getrs(a, piv, b)
-    dt = time.perf_counter() - t
+    return (time.perf_counter() - t) * 1e3
+
+
+def cpu_baseline(n_target, n_sample):
+    """The oracle's restatement of getrf_row_major + getrs (single thread) on an
+    n_sample Gaussian system, scaled to n_target by the n^3 flop ratio."""
+    ms = cpu_solve_ms(n_sample)
     scale = (n_target / n_sample) ** 3
-    return {"value": round(dt * scale * 1e3, 1), "unit": "ms",
+    return {"value": round(ms * scale, 1), "unit": "ms",
             "cores": 1, "kind": "port",
             "sample": f"oracle getrf_row_major+getrs on one core, n={n_sample} N(0,1), "
-                      f"{dt:.2f} s measured, scaled by (n/{n_sample})^3 to n={n_target}",
-            "cpu": platform.processor() or platform.machine()}
+                      f"{ms / 1e3:.2f} s measured, scaled by (n/{n_sample})^3 to n={n_target}",
+            "cpu": cpu_model(), "nproc": os.cpu_count()}
+
+
+def bench_plain(ctx, gl, n, nrhs, steps, nb, seed):
+    """Plain FP64 getrf + getrs from device-resident N(0,1) data (the C1 and
+    C2 shapes): ms per factor-and-solve, with its backward error."""
+    import lair_amd as L
+    lib = ctx.lib
+    ld = n + 64 if n % 4096 == 0 else ((n + 63) // 64) * 64
+    da, dlu = L.DeviceArray(n, n, ld=ld), L.DeviceArray(n, n, ld=ld)
+    db, dx = L.DeviceArray(n, nrhs), L.DeviceArray(n, nrhs)
+    dp = L.DeviceArray(n, 1, np.int64)
+    rc = gl.lair_gen_gaussian(da.ptr, n, n, ld, seed)
+    rc |= gl.lair_gen_gaussian(db.ptr, n, nrhs, nrhs, seed + 1)
+    if rc:
+        raise RuntimeError("input generation failed")
+    sing = ctypes.c_int64(-1)
+
+    def step():
+        lib.lair_memcpy_d2d(dlu.ptr, da.ptr, n * ld * 8)
+        lib.lair_memcpy_d2d(dx.ptr, db.ptr, n * nrhs * 8)
+        L.check(lib.lair_dgetrf_device(ctx.h, n, n, dlu.ptr, ld, dp.ptr, nb, ctypes.byref(sing)), "getrf")
+        L.check(lib.lair_dgetrs_device(ctx.h, n, nrhs, dlu.ptr, ld, dp.ptr, dx.ptr, nrhs, 0), "getrs")
+
+    step()  # warm-up (and the autotuner's one-time search when nb < 0)
+    times = []
+    for _ in range(steps):
+        lib.lair_memcpy_d2d(dlu.ptr, da.ptr, n * ld * 8)
+        lib.lair_memcpy_d2d(dx.ptr, db.ptr, n * nrhs * 8)
+        ctx.synchronize()
+        t = time.perf_counter()
+        L.check(lib.lair_dgetrf_device(ctx.h, n, n, dlu.ptr, ld, dp.ptr, nb, ctypes.byref(sing)), "getrf")
+        L.check(lib.lair_dgetrs_device(ctx.h, n, nrhs, dlu.ptr, ld, dp.ptr, dx.ptr, nrhs, 0), "getrs")
+        ctx.synchronize()
+        times.append(time.perf_counter() - t)
+    v = ctypes.c_double()
+    L.check(lib.lair_dbackward_error(ctx.h, n, nrhs, da.ptr, ld, db.ptr, nrhs, dx.ptr, nrhs, ctypes.byref(v)),
+            "berr")
+    for d in (da, dlu, db, dx, dp):
+        d.free()
+    ms = float(np.median(times)) * 1e3
+    flops = 2.0 * n ** 3 / 3 + 2.0 * n * n * nrhs
+    return {"ms": round(ms, 3), "n": n, "nrhs": nrhs, "panel_width": "autotuned" if nb < 0 else nb,
+            "backward_error": v.value, "tflops": round(flops / (ms * 1e-3) / 1e12, 2)}
 
 
 def main():
@@ -241,6 +307,10 @@ def main():
                     help="A ~ N(0,1) from the library's own generator kernel instead of the kappa=1e6 "
                          "construction (same factorization shapes; for counter-collection runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain FP64 getrf + getrs timings at n=512 and n=8192")
+    ap.add_argument("--no-nrhs64", action="store_true",
+                    help="skip the 64-right-hand-side run of the headline workload")
     ap.add_argument("--only-batched", action="store_true",
                     help="only the batched (C4) measurements, FP64 then FP32, one JSON line each "
                          "(profiling runs of the batched kernel)")
@@ -334,8 +404,9 @@ def main():
         for _ in range(W):
             step()
         lib.lair_profile_reset(ctx.h)
-        # Events around the trailing-update kernels only (the dominant class).
-        lib.lair_profile_enable(ctx.h, 1 << _capi.KCLASS["gemm"])
+        # Events around the panel leaves (the critical path's dominant kernel)
+        # and the trailing updates, on the streams they are launched on.
+        lib.lair_profile_enable(ctx.h, (1 << _capi.KCLASS["gemm"]) | (1 << _capi.KCLASS["panel"]))
         ctx.synchronize()
         t = time.perf_counter()
         for _ in range(K):
@@ -344,10 +415,12 @@ def main():
         ms = (time.perf_counter() - t) * 1e3 / K
         lib.lair_profile_enable(ctx.h, 0)
         g_ms, g_cnt, g_fl, g_by = prof_read(ctx, "gemm")
+        p_ms, p_cnt, p_fl, p_by = prof_read(ctx, "panel")
         be = berr()
         results[name] = {"ms": ms, "status": st, "iterations": info.iterations,
                          "converged": info.converged, "fell_back": info.fell_back,
-                         "backward_error": be, "gemm": (g_ms / K, g_cnt // K, g_fl / K, g_by / K)}
+                         "backward_error": be, "gemm": (g_ms / K, g_cnt // K, g_fl / K, g_by / K),
+                         "panel": (p_ms / K, p_cnt // K, p_fl / K, p_by / K)}
         log(f"{name}: {ms:.2f} ms/solve, {info.iterations} IR steps, backward error {be:.2e}, "
             f"fell_back={info.fell_back}, trailing gemm {g_ms / K:.2f} ms over {g_cnt // K} launches")
 
@@ -363,6 +436,18 @@ def main():
     lib.lair_profile_enable(ctx.h, 0)
     breakdown = {k: round(prof_read(ctx, k)[0], 3) for k in _capi.KCLASS}
 
+    # Roofline of the panel leaves (the kernel class that dominates the
+    # critical path): algorithmic bytes = the leaf read once and written once,
+    # 2 * rows * width * 4 bytes per launch, over the launches' event time.
+    p_ms, p_cnt, p_fl, p_by = hr["panel"]
+    p_ach = p_by / (p_ms * 1e-3) / 1e9 if p_ms > 0 else 0.0
+    roof = {"bound": "hbm", "achieved": round(p_ach, 2), "peak": PEAK["hbm_gbs"], "unit": "GB/s",
+            "frac": round(p_ach / PEAK["hbm_gbs"], 5), "traffic": measured_traffic("panel"),
+            "kernel": f"resident panel leaf rpanel_kernel ({head}); latency-bound by the per-column "
+                      f"exchange", "launches_per_step": p_cnt,
+            "algorithmic_bytes_per_launch": round(p_by / max(p_cnt, 1)),
+            "us_per_launch": round(p_ms * 1e3 / max(p_cnt, 1), 1),
+            "us_per_column": round(p_ms * 1e3 / n, 3), "ms_per_step": round(p_ms, 3)}
     # Roofline of the trailing update: the lower of the MFMA and HBM ceilings.
     g_ms, g_cnt, g_fl, g_by = hr["gemm"]
     mfma_peak = PEAK["bf16"] if head.startswith("bf16") else PEAK["f32"]
@@ -371,15 +456,51 @@ def main():
     t_hbm = g_by / (PEAK["hbm_gbs"] * 1e9)
     if t_hbm >= t_mfma:
         achieved = g_by / (g_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK["hbm_gbs"], "unit": "GB/s",
-                "frac": round(achieved / PEAK["hbm_gbs"], 4)}
+        roof_g = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK["hbm_gbs"], "unit": "GB/s",
+                  "frac": round(achieved / PEAK["hbm_gbs"], 4)}
     else:
         achieved = g_fl / (g_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": mfma_peak, "unit": "TFLOP/s",
-                "frac": round(achieved / mfma_peak, 4)}
-    roof.update({"traffic": measured_traffic(head), "kernel": f"trailing update ({head})",
-                 "launches_per_step": g_cnt, "algorithmic_flops_per_step": g_fl,
-                 "algorithmic_bytes_per_step": g_by, "ms_per_step": round(g_ms, 3)})
+        roof_g = {"bound": "mfma", "achieved": round(achieved, 2), "peak": mfma_peak, "unit": "TFLOP/s",
+                  "frac": round(achieved / mfma_peak, 4)}
+    roof_g.update({"traffic": measured_traffic(head), "kernel": f"trailing update ({head})",
+                   "launches_per_step": g_cnt, "algorithmic_flops_per_step": g_fl,
+                   "algorithmic_bytes_per_step": g_by, "ms_per_step": round(g_ms, 3)})
+
+    # The same workload with 64 right-hand sides: FP64 and the fastest variant.
+    c3_64 = None
+    if not args.no_nrhs64 and not args.only_mixed and nrhs == 1:
+        r64 = 64
+        db64, dx64 = L.DeviceArray(n, r64), L.DeviceArray(n, r64)
+        if gl.lair_gen_gaussian(db64.ptr, n, r64, r64, 16386):
+            raise RuntimeError("input generation failed")
+
+        def berr64():
+            v = ctypes.c_double()
+            L.check(lib.lair_dbackward_error(ctx.h, n, r64, da.ptr, ld, db64.ptr, r64, dx64.ptr, r64,
+                                             ctypes.byref(v)), "berr")
+            return v.value
+
+        def timed(fn):
+            fn()
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(K):
+                fn()
+            ctx.synchronize()
+            return (time.perf_counter() - t0) * 1e3 / K
+
+        f64 = timed(lambda: L.check(lib.lair_dgesv_device(ctx.h, n, r64, da.ptr, ld, db64.ptr, r64, dx64.ptr,
+                                                          r64), "dgesv"))
+        f64_be = berr64()
+        info64 = _capi.IrInfo()
+        mx = timed(lambda: lib.lair_dgesv_mixed_device(ctx.h, n, r64, da.ptr, ld, db64.ptr, r64, dx64.ptr, r64,
+                                                       PREC_NAMES[head], 20, 0.0, ctypes.byref(info64)))
+        c3_64 = {"nrhs": r64, "fp64_ms": round(f64, 3), "mixed_ms": round(mx, 3), "precision": head,
+                 "speedup_vs_fp64": round(f64 / mx, 3), "ir_iterations": info64.iterations,
+                 "fell_back": info64.fell_back, "backward_error": berr64(), "fp64_backward_error": f64_be}
+        log(f"nrhs=64: fp64 {f64:.2f} ms, {head} {mx:.2f} ms ({info64.iterations} IR steps)")
+        db64.free()
+        dx64.free()
 
     side = {}
     if not args.no_side and not args.only_mixed:
@@ -392,10 +513,22 @@ def main():
         side["adjoint"] = bench_adjoint(ctx, gl, max(K, 3))
         log(f"adjoint n=4096 nrhs=64: {side['adjoint']['ms']:.3f} ms")
 
+    plain = {}
+    if not args.no_plain and not args.only_mixed:
+        plain["c1"] = bench_plain(ctx, gl, 512, 1, max(K, 5), 0, 512)
+        plain["c2"] = bench_plain(ctx, gl, 8192, 16, max(K, 3), -1, 8192)
+        log(f"plain FP64 getrf+getrs: n=512 {plain['c1']['ms']:.3f} ms, n=8192 nrhs=16 {plain['c2']['ms']:.2f} ms")
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
             cpu = cpu_baseline(n, args.cpu_sample_n)
+            if "c1" in plain:
+                c1_cpu = cpu_solve_ms(512, 512)
+                plain["c1"]["cpu_ms_one_core"] = round(c1_cpu, 2)
+                plain["c1"]["speedup_vs_cpu"] = round(c1_cpu / plain["c1"]["ms"], 1)
+            if "c2" in plain:
+                plain["c2"]["cpu_ms_one_core_extrapolated"] = round(cpu["value"] * (8192 / n) ** 3, 1)
         except Exception as e:  # the oracle build is optional on a box without gcc
             log(f"cpu baseline skipped: {e}")
 
@@ -426,11 +559,15 @@ def main():
                          for k, v in results.items()},
             "h2d_ms_A": round(h2d_ms, 2),
             "breakdown_ms": breakdown,
+            "c3_nrhs64": c3_64,
+            "c1": plain.get("c1"),
+            "c2": plain.get("c2"),
             "batched_c4": side.get("batched_fp64"),
             "batched_c4_fp32": side.get("batched_fp32"),
             "adjoint_c5": side.get("adjoint"),
         },
         "roofline": roof,
+        "roofline_gemm": roof_g,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <numeric>
 #include <vector>
@@ -391,21 +392,33 @@ struct SolveAux {
   const T *uinv = nullptr;
 };
 
-// Builds SolveAux in the context's workspace.  perm is taken from perm_in
-// when the factorization produced it, otherwise composed from ipiv.
+// Device buffers of a getrs that the caller provides instead of the
+// context's grow-only workspace (the allocation-free entry point).
+template <typename T>
+struct GetrsScratch {
+  int *perm = nullptr;
+  T *linv = nullptr, *uinv = nullptr, *tmp = nullptr;
+};
+
+// Builds SolveAux in the context's workspace (or in sc's buffers).  perm is
+// taken from perm_in when the factorization produced it, otherwise composed
+// from ipiv.
 template <typename T>
 static lair_status make_aux(lair_context *ctx, int n, const T *lu, int64_t lda, const int64_t *ipiv,
-                            const int *perm_in, SolveAux<T> *aux) {
+                            const int *perm_in, SolveAux<T> *aux,
+                            const GetrsScratch<T> *sc = nullptr) {
   constexpr int BS = TrsvBlock<T>::BS;
   const int nblk = (n + BS - 1) / BS;
-  LAIR_HIP(ensure(ctx->ws_linv, sizeof(T) * (size_t)nblk * BS * BS));
-  LAIR_HIP(ensure(ctx->ws_uinv, sizeof(T) * (size_t)nblk * BS * BS));
-  T *linv = static_cast<T *>(ctx->ws_linv.ptr);
-  T *uinv = static_cast<T *>(ctx->ws_uinv.ptr);
+  if (!sc) {
+    LAIR_HIP(ensure(ctx->ws_linv, sizeof(T) * (size_t)nblk * BS * BS));
+    LAIR_HIP(ensure(ctx->ws_uinv, sizeof(T) * (size_t)nblk * BS * BS));
+  }
+  T *linv = sc ? sc->linv : static_cast<T *>(ctx->ws_linv.ptr);
+  T *uinv = sc ? sc->uinv : static_cast<T *>(ctx->ws_uinv.ptr);
   hipError_t e;
   if (!perm_in) {
-    LAIR_HIP(ensure(ctx->ws_perm, sizeof(int) * (size_t)n));
-    int *perm = static_cast<int *>(ctx->ws_perm.ptr);
+    if (!sc) LAIR_HIP(ensure(ctx->ws_perm, sizeof(int) * (size_t)n));
+    int *perm = sc ? sc->perm : static_cast<int *>(ctx->ws_perm.ptr);
     e = launch_perm_from_ipiv(ctx, ipiv, n, perm);
     if (e != hipSuccess) return hip_status(e, "perm");
     perm_in = perm;
@@ -422,9 +435,10 @@ static lair_status make_aux(lair_context *ctx, int n, const T *lu, int64_t lda, 
 // A^T X = B as U^T forward, L^T backward, scatter (P^T).
 template <typename T>
 static lair_status getrs_aux(lair_context *ctx, int n, int nrhs, const T *lu, int64_t lda,
-                             const SolveAux<T> &aux, T *b, int64_t ldb, int trans) {
-  LAIR_HIP(ensure(ctx->ws_tmp2, sizeof(T) * (size_t)n * nrhs));
-  T *t = static_cast<T *>(ctx->ws_tmp2.ptr);
+                             const SolveAux<T> &aux, T *b, int64_t ldb, int trans,
+                             const GetrsScratch<T> *sc = nullptr) {
+  if (!sc) LAIR_HIP(ensure(ctx->ws_tmp2, sizeof(T) * (size_t)n * nrhs));
+  T *t = sc ? sc->tmp : static_cast<T *>(ctx->ws_tmp2.ptr);
   hipError_t e;
   if (!trans) {
     e = launch_permute_rows<T, T>(ctx, n, nrhs, aux.perm, b, ldb, t, nrhs, false);
@@ -457,7 +471,8 @@ static lair_status getrs_aux(lair_context *ctx, int n, int nrhs, const T *lu, in
 // the dataflow solve.
 template <typename T>
 static lair_status getrs_impl(lair_context *ctx, int64_t n64, int64_t nrhs64, const T *lu,
-                              int64_t lda, const int64_t *ipiv, T *b, int64_t ldb, int trans) {
+                              int64_t lda, const int64_t *ipiv, T *b, int64_t ldb, int trans,
+                              const GetrsScratch<T> *sc = nullptr) {
   if (n64 < 0) return -2;
   if (nrhs64 < 0) return -3;
   if (n64 == 0 || nrhs64 == 0) return LAIR_OK;
@@ -467,9 +482,9 @@ static lair_status getrs_impl(lair_context *ctx, int64_t n64, int64_t nrhs64, co
     return hip_status(e, "getrs");
   }
   SolveAux<T> aux;
-  lair_status st = make_aux<T>(ctx, n, lu, lda, ipiv, nullptr, &aux);
+  lair_status st = make_aux<T>(ctx, n, lu, lda, ipiv, nullptr, &aux, sc);
   if (st != LAIR_OK) return st;
-  return getrs_aux<T>(ctx, n, nrhs, lu, lda, aux, b, ldb, trans);
+  return getrs_aux<T>(ctx, n, nrhs, lu, lda, aux, b, ldb, trans, sc);
 }
 
 static double bits_to_double(unsigned long long b) {

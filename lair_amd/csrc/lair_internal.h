@@ -127,6 +127,12 @@ struct lair_context {
   // Grow-only workspaces.
   lair::Buffer ws_a32, ws_lu64, ws_ipiv, ws_x32, ws_r, ws_tmp, ws_rhs;
   lair::Buffer ws_perm, ws_linv, ws_uinv, ws_flags, ws_tmp2;
+  // When set, the triangular solves publish their tagged values here instead
+  // of in ws_flags (caller-provided memory, zeroed by the caller).
+  unsigned long long *xt_override = nullptr;
+  // Device address, on this context's device, of the allocation-free getrs's
+  // scratch array (driver.hip).
+  void *noalloc_scratch = nullptr;
   size_t ws_flags_zeroed = 0;
   uint32_t tri_epoch = 0;
 

@@ -11,10 +11,13 @@
 //    B stripe in LDS, 16-row diagonal sub-blocks solved per column and the rows
 //    below updated in parallel.
 //  * trtri_diag: inverses of the diagonal blocks of L (unit) and U.
-//  * trsv_dataflow: block-row workgroups that wait on per-block flags; tiles
-//    are fetched before waiting, so the sequential chain only carries one
-//    flag round trip, one tile product and one diagonal-inverse product per
-//    block (the substitutions of src/lapack/getrs.rs:24-36, blocked).
+//  * trsv_dataflow: block-row workgroups that wait for the solved values of
+//    the blocks they depend on; those values are published as tagged 64-bit
+//    words (tag = the solve's epoch), so a reader that sees its epoch in a
+//    word also has the value, with no flags and no fences.  Tiles are fetched
+//    before waiting, so the sequential chain only carries one round trip, one
+//    tile product and one diagonal-inverse product per block (the
+//    substitutions of src/lapack/getrs.rs:24-36, blocked).
 #include <climits>
 
 #include "device_util.h"
@@ -416,8 +419,8 @@ __global__ void __launch_bounds__(256)
 // first kb entries are rows k .. k + kb - 1 in order) are read once into LDS;
 // the rows moved below the panel are written back at once, and the top kb
 // rows are solved in LDS and written as U12.  Diagonal 16-row blocks are
-// solved in registers by one thread per column; the rows below each block
-// are then updated by all threads.  Every element receives its subtractions
+// solved in registers, 16 lanes per column; the rows below each block are
+// then updated by all threads.  Every element receives its subtractions
 // b -= b[p] * L[i][p] one at a time in increasing p (src/blas/trsm.rs:13-21
 // applies them in the same order).
 template <typename T>
@@ -427,6 +430,7 @@ __global__ void __launch_bounds__(256)
                      const int *__restrict__ count, int nmax) {
 #pragma clang fp contract(off)
   constexpr int CW = kTrsmCW, LX = CW + 1, RG = 256 / CW;
+  static_assert(CW == 16, "the diagonal solve maps 4 waves x 4 columns x 16 rows onto 256 threads");
   extern __shared__ unsigned char smem_raw[];
   T *sL = reinterpret_cast<T *>(smem_raw);             // strictly lower L11, row r at r(r-1)/2
   T *sX = sL + ((size_t)kb * (kb - 1) / 2 + 1);         // [nmax][CW + 1] map rows' old values
@@ -477,24 +481,26 @@ __global__ void __launch_bounds__(256)
     if (c < nc && s_rows[e] != s_src[e]) a[(int64_t)s_rows[e] * lda + cs + c] = sX[e * LX + c];
   for (int b0 = 0; b0 < kb; b0 += 16) {
     const int bs = kb - b0 < 16 ? kb - b0 : 16;
-    if (t < CW) {
-      T x[16];
+    {
+      // Diagonal block: 16 lanes per column, lane rr holding row b0 + rr of
+      // column cc.  At step i row i is final; it is broadcast to the column's
+      // lanes and the rows below it subtract their term.
+      const int l = t & 63;
+      const int cc = (t >> 6) * 4 + (l >> 4), rr = l & 15;
+      T x = rr < bs ? sX[(b0 + rr) * LX + cc] : T(0);
+      T lrow[15];
 #pragma unroll
-      for (int i = 0; i < 16; ++i) x[i] = i < bs ? sX[(b0 + i) * LX + t] : T(0);
+      for (int i = 0; i < 15; ++i)
+        lrow[i] = (i < rr && rr < bs) ? sL[(b0 + rr) * (b0 + rr - 1) / 2 + b0 + i] : T(0);
 #pragma unroll
-      for (int i = 1; i < 16; ++i) {
-        if (i < bs) {
-          const T *lr = sL + (b0 + i) * (b0 + i - 1) / 2 + b0;
-#pragma unroll
-          for (int p = 0; p < i; ++p) {
-            const T e = x[p] * lr[p];
-            x[i] = x[i] - e;
-          }
+      for (int i = 0; i < 15; ++i) {
+        const T xi = __shfl(x, (l & ~15) | i);
+        if (i < rr && rr < bs) {
+          const T e = xi * lrow[i];
+          x = x - e;
         }
       }
-#pragma unroll
-      for (int i = 0; i < 16; ++i)
-        if (i < bs) sX[(b0 + i) * LX + t] = x[i];
+      if (rr < bs) sX[(b0 + rr) * LX + cc] = x;
     }
     __syncthreads();
     T xb[16];
@@ -662,7 +668,7 @@ struct TrsvArgs {
   T *x;           // n x nrhs solution
   int64_t ldx;
   int n, nrhs, nblk, nchunks;
-  uint32_t *flags;  // [nblk][nchunks]
+  unsigned long long *xt;  // [n][nrhs][words per element] tagged solved values
   uint32_t epoch;
   int forward, trans, dinv_trans;
   int32_t *err;
@@ -678,6 +684,7 @@ struct TrsvArgs {
 // wait for the previous block) -> LDS -> row segments per thread.
 template <typename T, int BS, int RC>
 __global__ void __launch_bounds__(256) trsv_dataflow_kernel(TrsvArgs<T> p) {
+  constexpr int WPE = sizeof(T) / 4;  // 32-bit words per element
   constexpr int TPR = 256 / BS;   // threads per row in the product
   constexpr int SEG = BS / TPR;   // columns per thread
   constexpr int VEC = 16 / sizeof(T);           // elements per 16-byte load
@@ -771,33 +778,49 @@ __global__ void __launch_bounds__(256) trsv_dataflow_kernel(TrsvArgs<T> p) {
     const int cb = j * BS;
     stash(stage, p.trans != 0);
     if (d + 1 < nb) issue(d + 1, stage);
-    if (t == 0) {
-      // Acquire: the block's x values, stored before its release below, are
-      // visible once its flag reaches this solve's epoch.
-      uint32_t *f = p.flags + (size_t)j * p.nchunks + ch;
-      int spins = 0;
-      for (;;) {
-        if (spins >= p.wait_limit) {
-          atomic_or_agent(p.err, kErrWait);
-          s_bail = 1;
-          break;
+    // The solved values of block j: every thread polls its own tagged words
+    // (bounded wait).
+    {
+      bool bail = false;
+      for (int e = t; e < BS * RC; e += 256) {
+        const int rr = e / RC, q = e % RC;
+        const int gi = cb + rr;
+        T v = T(0);
+        if (gi < p.n && q < ncols) {
+          const unsigned long long *w = p.xt + ((int64_t)gi * p.nrhs + c0 + q) * WPE;
+          int spins = 0;
+          for (;;) {
+            if (spins >= p.wait_limit) {
+              bail = true;
+              break;
+            }
+            uint32_t part[WPE];
+            bool ok = true;
+#pragma unroll
+            for (int k = 0; k < WPE; ++k) {
+              const unsigned long long x = ld_sc1(w + k);
+              ok = ok && (uint32_t)(x >> 32) == p.epoch;
+              part[k] = (uint32_t)x;
+            }
+            if (ok) {
+              v = from_words<T>(part);
+              break;
+            }
+            if ((++spins & 63) == 0 && ld_sc1(p.err) != 0) {
+              bail = true;
+              break;
+            }
+          }
         }
-        if ((int32_t)(__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) - p.epoch) >= 0)
-          break;
-        if ((++spins & 63) == 0 && ld_sc1(p.err) != 0) {
-          s_bail = 1;
-          break;
-        }
+        xs[rr][q] = v;
+      }
+      if (bail) {
+        atomic_or_agent(p.err, kErrWait);
+        s_bail = 1;
       }
     }
     __syncthreads();
     if (s_bail) return;
-    for (int e = t; e < BS * RC; e += 256) {
-      const int rr = e / RC, q = e % RC;
-      const int gi = cb + rr;
-      xs[rr][q] = (gi < p.n && q < ncols) ? ld_sc1(p.x + (int64_t)gi * p.ldx + c0 + q) : T(0);
-    }
-    __syncthreads();
 #pragma unroll
     for (int s = 0; s < SEG; ++s) {
       const int c = h * SEG + s;
@@ -837,17 +860,15 @@ __global__ void __launch_bounds__(256) trsv_dataflow_kernel(TrsvArgs<T> p) {
   }
   if (h == 0 && rowok) {
 #pragma unroll
-    for (int q = 0; q < RC; ++q)
-      if (q < ncols) st_sc1(p.x + (int64_t)gr * p.ldx + c0 + q, out[q]);
+    for (int q = 0; q < RC; ++q) {
+      if (q < ncols) {
+        p.x[(int64_t)gr * p.ldx + c0 + q] = out[q];
+        unsigned long long *w = p.xt + ((int64_t)gr * p.nrhs + c0 + q) * WPE;
+#pragma unroll
+        for (int k = 0; k < WPE; ++k) st_sc1(w + k, tagged(p.epoch, word_of<T>(out[q], k)));
+      }
+    }
   }
-  // Every thread releases its x stores at agent scope (each wave waits for
-  // its own stores), the barrier orders them before thread 0's flag store,
-  // and that release store publishes them to the dependent blocks.
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  __syncthreads();
-  if (t == 0)
-    __hip_atomic_store(p.flags + (size_t)i * p.nchunks + ch, p.epoch, __ATOMIC_RELEASE,
-                       __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <typename T>
@@ -862,13 +883,25 @@ hipError_t launch_trsv_dataflow(lair_context *ctx, const T *lu, int64_t lda, con
   const int rc = nrhs == 1 ? 1 : 4;
   const int nblk = (n + BS - 1) / BS;
   const int nchunks = (nrhs + rc - 1) / rc;
-  hipError_t e = ensure(ctx->ws_flags, sizeof(uint32_t) * (size_t)nblk * nchunks);
-  if (e != hipSuccess) return e;
-  if (ctx->ws_flags_zeroed < ctx->ws_flags.bytes || ctx->tri_epoch > 0x70000000u) {
-    e = launch_fill_u32(ctx, static_cast<uint32_t *>(ctx->ws_flags.ptr),
-                        (int64_t)(ctx->ws_flags.bytes / sizeof(uint32_t)), 0u);
+  // Tagged words of the solved values; zeroed whenever they are (re)allocated
+  // or the epoch would wrap, so that no stale word carries a live tag.  A
+  // caller-provided buffer (xt_override) was zeroed by the caller.
+  unsigned long long *xt = ctx->xt_override;
+  if (!xt) {
+    const size_t xt_bytes = sizeof(unsigned long long) * (size_t)n * nrhs * (sizeof(T) / 4);
+    const bool grow = ctx->ws_flags.bytes < xt_bytes;
+    hipError_t e = ensure(ctx->ws_flags, xt_bytes);
     if (e != hipSuccess) return e;
-    ctx->ws_flags_zeroed = ctx->ws_flags.bytes;
+    if (grow) ctx->ws_flags_zeroed = 0;
+    if (ctx->ws_flags_zeroed < ctx->ws_flags.bytes || ctx->tri_epoch > 0x70000000u) {
+      e = launch_fill_u32(ctx, static_cast<uint32_t *>(ctx->ws_flags.ptr),
+                          (int64_t)(ctx->ws_flags.bytes / sizeof(uint32_t)), 0u);
+      if (e != hipSuccess) return e;
+      ctx->ws_flags_zeroed = ctx->ws_flags.bytes;
+      ctx->tri_epoch = 0;
+    }
+    xt = static_cast<unsigned long long *>(ctx->ws_flags.ptr);
+  } else if (ctx->tri_epoch > 0x70000000u) {
     ctx->tri_epoch = 0;
   }
   TrsvArgs<T> a;
@@ -882,7 +915,7 @@ hipError_t launch_trsv_dataflow(lair_context *ctx, const T *lu, int64_t lda, con
   a.nrhs = nrhs;
   a.nblk = nblk;
   a.nchunks = nchunks;
-  a.flags = static_cast<uint32_t *>(ctx->ws_flags.ptr);
+  a.xt = xt;
   a.epoch = ++ctx->tri_epoch;
   a.forward = forward ? 1 : 0;
   a.trans = trans ? 1 : 0;
