@@ -159,6 +159,21 @@ lair_status lair_sgetrs_device(lair_context *ctx, int64_t n, int64_t nrhs, const
                                int64_t lda, const int64_t *ipiv, float *b, int64_t ldb,
                                int32_t trans);
 
+/* lair_dgetrs_device without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate (code under stream capture,
+ * tight serving loops).  The workspace (row permutation, inverted diagonal
+ * blocks, staging copy, tagged solved values) lives in a static device array
+ * of 22.1 MiB held by every GPU the library runs on for the life of the
+ * process; n <= 16384 and nrhs <= 16, larger calls return
+ * LAIR_ERR_UNSUPPORTED.  Every call on one GPU shares that array, so calls
+ * that overlap on one GPU (two contexts, two host threads) race; calls on
+ * one context are serialized.  Same results as lair_dgetrs_device;
+ * synchronizes.  With profiling enabled (lair_profile_enable) the timing
+ * events it records are created on first use. */
+lair_status lair_dgetrs_device_noalloc(lair_context *ctx, int64_t n, int64_t nrhs, const double *lu,
+                                       int64_t lda, const int64_t *ipiv, double *b, int64_t ldb,
+                                       int32_t trans);
+
 /* Result record of the mixed-precision solver. */
 typedef struct lair_ir_info {
   int32_t iterations;   /* refinement steps taken */

@@ -69,6 +69,7 @@ SIGNATURES = [
     ("lair_sgetrf_device", i32, [vp, i64, i64, vp, i64, vp, i64, P64]),
     ("lair_sgetrf_pivot_device", i32, [vp, i64, i64, vp, i64, vp, i64, i32, i32, P64]),
     ("lair_dgetrs_device", i32, [vp, i64, i64, vp, i64, vp, vp, i64, i32]),
+    ("lair_dgetrs_device_noalloc", i32, [vp, i64, i64, vp, i64, vp, vp, i64, i32]),
     ("lair_sgetrs_device", i32, [vp, i64, i64, vp, i64, vp, vp, i64, i32]),
     ("lair_dgesv_mixed_device", i32,
      [vp, i64, i64, vp, i64, vp, i64, vp, i64, i32, i32, f64, ctypes.POINTER(IrInfo)]),

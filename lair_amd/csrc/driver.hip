@@ -530,6 +530,43 @@ static lair_status backward_error_from(lair_context *ctx, int n, int nrhs, const
 
 using namespace lair;
 
+// ---------------------------------------------- allocation-free getrs ---
+
+// Scratch of lair_dgetrs_device_noalloc: the row permutation, the inverses of
+// the diagonal blocks of L and U, a staging copy of the right-hand sides and
+// the triangular solves' tagged values, for n <= kNoAllocMaxN and
+// nrhs <= kNoAllocMaxRhs (22.1 MiB).  One copy per GPU, for the life of the
+// process; host code uses the per-device address from hipGetSymbolAddress,
+// never the array's name.
+constexpr int64_t kNoAllocMaxN = 16384;
+constexpr int64_t kNoAllocMaxRhs = 16;
+constexpr size_t kNoAllocPermBytes = sizeof(int) * kNoAllocMaxN;
+constexpr size_t kNoAllocInvBytes = sizeof(double) * kNoAllocMaxN * TrsvBlock<double>::BS;
+constexpr size_t kNoAllocTmpBytes = sizeof(double) * kNoAllocMaxN * kNoAllocMaxRhs;
+constexpr size_t kNoAllocXtBytes = 2 * sizeof(unsigned long long) * kNoAllocMaxN * kNoAllocMaxRhs;
+constexpr size_t kNoAllocBytes = kNoAllocPermBytes + 2 * kNoAllocInvBytes + kNoAllocTmpBytes + kNoAllocXtBytes;
+static_assert(kNoAllocPermBytes % 256 == 0 && kNoAllocInvBytes % 256 == 0 && kNoAllocTmpBytes % 256 == 0,
+              "scratch regions stay 256-byte aligned");
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+static std::mutex g_scratch_mu;
+static std::map<int, void *> g_scratch_ptr;
+
+// The scratch array's address on `device`, which must be the current device.
+static lair_status noalloc_scratch_address(int device, void **out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  if (it != g_scratch_ptr.end()) {
+    *out = it->second;
+    return LAIR_OK;
+  }
+  void *p = nullptr;
+  LAIR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[device] = p;
+  *out = p;
+  return LAIR_OK;
+}
+
 // ------------------------------------------------------------------ C ABI
 
 static std::mutex g_default_mu;
@@ -636,6 +673,10 @@ lair_status lair_context_create(lair_context **out) {
   if (!out) return -1;
   std::unique_ptr<lair_context> ctx(new lair_context());
   LAIR_HIP(hipGetDevice(&ctx->device));
+  {
+    const lair_status st = noalloc_scratch_address(ctx->device, &ctx->noalloc_scratch);
+    if (st != LAIR_OK) return st;
+  }
   LAIR_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   LAIR_HIP(hipMalloc(&ctx->rec, sizeof(unsigned long long) * 2 * kMaxPanelWG * 64));
   LAIR_HIP(hipMalloc(&ctx->drec, sizeof(unsigned long long) * 2 * 64));
@@ -817,6 +858,40 @@ lair_status lair_sgetrs_device(lair_context *ctx, int64_t n, int64_t nrhs, const
   if (!ctx) return -1;
   std::lock_guard<std::mutex> lk(ctx->mu);
   lair_status st = getrs_impl<float>(ctx, n, nrhs, lu, lda, ipiv, b, ldb, trans);
+  return st != LAIR_OK ? st : sync_status(ctx);
+}
+
+lair_status lair_dgetrs_device_noalloc(lair_context *ctx, int64_t n, int64_t nrhs, const double *lu,
+                                       int64_t lda, const int64_t *ipiv, double *b, int64_t ldb,
+                                       int32_t trans) {
+  if (!ctx) return -1;
+  if (n < 0) return -2;
+  if (nrhs < 0) return -3;
+  if (n > kNoAllocMaxN || nrhs > kNoAllocMaxRhs) {
+    set_error("lair_dgetrs_device_noalloc: n <= 16384 and nrhs <= 16 (the size of its scratch)");
+    return LAIR_ERR_UNSUPPORTED;
+  }
+  if (!ctx->noalloc_scratch) {
+    set_error("lair_dgetrs_device_noalloc: no scratch address on this device");
+    return LAIR_ERR_DEVICE;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (n == 0 || nrhs == 0) return LAIR_OK;
+  char *base = static_cast<char *>(ctx->noalloc_scratch);
+  GetrsScratch<double> sc;
+  sc.perm = reinterpret_cast<int *>(base);
+  sc.linv = reinterpret_cast<double *>(base + kNoAllocPermBytes);
+  sc.uinv = reinterpret_cast<double *>(base + kNoAllocPermBytes + kNoAllocInvBytes);
+  sc.tmp = reinterpret_cast<double *>(base + kNoAllocPermBytes + 2 * kNoAllocInvBytes);
+  unsigned long long *xt =
+      reinterpret_cast<unsigned long long *>(base + kNoAllocPermBytes + 2 * kNoAllocInvBytes + kNoAllocTmpBytes);
+  // The tagged values start from zero on every call: the array keeps the
+  // last call's contents, possibly from another context on this GPU.
+  hipError_t e = launch_fill_u32(ctx, reinterpret_cast<uint32_t *>(xt), 4 * n * nrhs, 0u);
+  if (e != hipSuccess) return hip_status(e, "fill");
+  ctx->xt_override = xt;
+  const lair_status st = getrs_impl<double>(ctx, n, nrhs, lu, lda, ipiv, b, ldb, trans, &sc);
+  ctx->xt_override = nullptr;
   return st != LAIR_OK ? st : sync_status(ctx);
 }
 
