@@ -257,6 +257,50 @@ lair_status lair_dsolve_backward_device(lair_context *ctx, int64_t n, int64_t nr
 lair_status lair_dsolve_backward(int64_t n, int64_t nrhs, const double *lu, const int64_t *ipiv,
                                  const double *x, const double *xbar, double *bbar, double *abar);
 
+/* ------------------------------------------------ device-resident LU --- */
+
+/* lu::Factorized (src/decomposition/lu.rs:12-20) with its factors kept on the
+ * device: the packed LU, the pivots, the row permutation the factorization
+ * built and, for square systems (from the first solve on), the inverses of
+ * the diagonal blocks of L and U, so that repeated solves and adjoints reuse
+ * them.  FP64. */
+typedef struct lair_lu lair_lu;
+
+/* Replaces From<ArrayBase> for Factorized (lu.rs:156-171): factors a copy of
+ * the device matrix a (m x n, row-major, lda); nb as lair_dgetrf_device.
+ * Synchronizes. */
+lair_status lair_dlu_factor(lair_context *ctx, int64_t m, int64_t n, const double *a, int64_t lda, int64_t nb,
+                            lair_lu **lu);
+lair_status lair_dlu_destroy(lair_lu *lu);
+/* Shape and singular column (-1 for None): Factorized::is_singular
+ * (lu.rs:75-77) is singular >= 0. */
+lair_status lair_dlu_info(const lair_lu *lu, int64_t *m, int64_t *n, int64_t *singular);
+/* Device pointers of the packed factors (leading dimension ld), the pivots
+ * (min(m, n) entries, LAPACK convention) and the row permutation
+ * (perm[i] = original row now at row i).  Owned by the handle. */
+lair_status lair_dlu_factors(const lair_lu *lu, const double **lu_dev, int64_t *ld, const int64_t **ipiv,
+                             const int32_t **perm);
+/* Replaces Factorized::solve (lu.rs:87-98), extended to nrhs columns and to
+ * A^T X = B (trans = 1): b (device, n x nrhs, ldb) is overwritten with X.
+ * LAIR_ERR_SHAPE for a non-square factorization.  Synchronizes. */
+lair_status lair_dlu_solve(lair_context *ctx, lair_lu *lu, int64_t nrhs, double *b, int64_t ldb, int32_t trans);
+/* lair_dsolve_backward_device on the handle's factors (no permutation
+ * rebuild, no diagonal re-inversion).  Synchronizes. */
+lair_status lair_dlu_solve_backward(lair_context *ctx, lair_lu *lu, int64_t nrhs, const double *x, int64_t ldx,
+                                    const double *xbar, int64_t ldxb, double *bbar, int64_t ldbb,
+                                    double *abar, int64_t ldab);
+/* Factor extraction on the device into out (row-major, ldo):
+ * LAIR_LU_P  m x m: P with P[perm[j]][j] = 1          (Factorized::p, lu.rs:28-39)
+ * LAIR_LU_L  m x k: unit lower, k = min(m, n)          (Factorized::l, lu.rs:42-57)
+ * LAIR_LU_U  k x n: upper                              (Factorized::u, lu.rs:60-72)
+ * LAIR_LU_PL m x k: P * L                              (Factorized::into_pl, lu.rs:107-153)
+ * Synchronizes. */
+#define LAIR_LU_P 0
+#define LAIR_LU_L 1
+#define LAIR_LU_U 2
+#define LAIR_LU_PL 3
+lair_status lair_dlu_extract(lair_context *ctx, const lair_lu *lu, int32_t which, double *out, int64_t ldo);
+
 /* ----------------------------------------------- instrumentation ------- */
 
 /* Kernel classes timed with HIP events on the context's stream while

@@ -184,16 +184,61 @@ class lu:
     """src/decomposition/lu.rs"""
 
     class Factorized:
-        """LU decomposition factors (lu.rs:12-20); construction factors a copy
-        of the matrix on the GPU (lu.rs:156-171)."""
+        """LU decomposition factors (lu.rs:12-20).  Construction factors a copy
+        of the matrix on the GPU (lu.rs:156-171).  For float64 the factors stay
+        on the device in a lair_lu handle (pivots, row permutation and, after
+        the first solve, the inverted diagonal blocks): p, l, u, into_pl and
+        solve run there.  float32 factors through lair_sgetrf and extracts on
+        the host."""
 
         def __init__(self, a):
-            self.lu = np.array(a, copy=True)
-            self.pivots, self.singular = getrf(self.lu)
+            a = np.asarray(a)
+            self._h = None
+            if a.dtype == np.float64 and a.ndim == 2 and a.size > 0:
+                ctx = _default_context()
+                lib = ctx.lib
+                m, n = a.shape
+                da = DeviceArray.from_host(np.ascontiguousarray(a))
+                h = ctypes.c_void_p()
+                check(lib.lair_dlu_factor(ctx.h, m, n, da.ptr, n, 0, ctypes.byref(h)), "lu_factor")
+                da.free()
+                self._h, self._ctx = h, ctx
+                sing = ctypes.c_int64()
+                check(lib.lair_dlu_info(h, None, None, ctypes.byref(sing)), "lu_info")
+                self.singular = None if sing.value < 0 else int(sing.value)
+                lu_p, ld, piv_p = ctypes.c_void_p(), ctypes.c_int64(), ctypes.c_void_p()
+                check(lib.lair_dlu_factors(h, ctypes.byref(lu_p), ctypes.byref(ld), ctypes.byref(piv_p), None),
+                      "lu_factors")
+                host = np.empty((m, ld.value), np.float64)
+                check(lib.lair_memcpy_d2h(ctypes.c_void_p(host.ctypes.data), lu_p, host.nbytes), "d2h")
+                self.lu = np.ascontiguousarray(host[:, :n])
+                self.pivots = np.empty(min(m, n), np.int64)
+                check(lib.lair_memcpy_d2h(ctypes.c_void_p(self.pivots.ctypes.data), piv_p, self.pivots.nbytes),
+                      "d2h")
+            else:
+                self.lu = np.array(a, copy=True)
+                self.pivots, self.singular = getrf(self.lu)
+
+        def __del__(self):
+            try:
+                if self._h:
+                    self._ctx.lib.lair_dlu_destroy(self._h)
+                    self._h = None
+            except Exception:
+                pass
+
+        def _extract(self, which, rows, cols):
+            out = DeviceArray(rows, cols)
+            check(self._ctx.lib.lair_dlu_extract(self._ctx.h, self._h, which, out.ptr, cols), "lu_extract")
+            h = out.download()
+            out.free()
+            return h
 
         def p(self):
             """lu.rs:28-39: P with P[perm[i], i] = 1, perm = laswp of 0..nrows."""
             nr = self.lu.shape[0]
+            if self._h:
+                return self._extract(_capi.LU_P, nr, nr)
             perm = np.arange(nr, dtype=np.int64)
             for i, piv in enumerate(self.pivots):
                 if piv != i:
@@ -206,6 +251,8 @@ class lu:
             """lu.rs:42-57: unit lower nrows x rank."""
             nr, nc = self.lu.shape
             rank = min(nr, nc)
+            if self._h:
+                return self._extract(_capi.LU_L, nr, rank)
             l = np.tril(self.lu[:, :rank], -1)
             l[np.arange(rank), np.arange(rank)] = 1
             return l
@@ -214,21 +261,42 @@ class lu:
             """lu.rs:60-72: upper rank x ncols."""
             nr, nc = self.lu.shape
             rank = min(nr, nc)
+            if self._h:
+                return self._extract(_capi.LU_U, rank, nc)
             return np.triu(self.lu[:rank, :])
 
         def is_singular(self):
+            """lu.rs:75-77."""
             return self.singular is not None
 
         def solve(self, b):
-            """lu.rs:87-98."""
+            """lu.rs:87-98 (b may also be 2-D: one right-hand side per column)."""
+            b = np.asarray(b)
             if b.shape[0] != self.lu.shape[0]:
                 raise ShapeError(f"b must have {self.lu.shape[0]} elements")
-            return getrs(self.lu, self.pivots, b)
+            if not self._h:
+                return getrs(self.lu, self.pivots, b)
+            n = self.lu.shape[0]
+            x = np.ascontiguousarray(np.asarray(b, np.float64).reshape(n, -1))
+            db = DeviceArray.from_host(x)
+            st = self._ctx.lib.lair_dlu_solve(self._ctx.h, self._h, x.shape[1], db.ptr, x.shape[1], 0)
+            if st == _capi.LAIR_ERR_SHAPE:
+                raise ShapeError(_capi.last_error())
+            check(st, "lu_solve")
+            out = db.download()
+            db.free()
+            return out.reshape(b.shape)
 
         def into_pl(self):
-            """lu.rs:107-153: P*L in the first min(m,n) columns, in place."""
+            """lu.rs:107-153: P*L in the first min(m,n) columns, U's columns
+            beyond them unchanged."""
             nr, nc = self.lu.shape
             k = min(nr, nc)
+            if self._h:
+                out = self.lu.copy()
+                out[:, :k] = self._extract(_capi.LU_PL, nr, k)
+                self.lu = out
+                return out
             pivots = list(self.pivots) + list(range(len(self.pivots), nr))
             for i in range(len(pivots) - 1, -1, -1):
                 target = pivots[i]
@@ -259,6 +327,17 @@ class lu:
                 else:
                     dst = src
             return self.lu
+
+
+_default = None
+
+
+def _default_context():
+    """A context shared by the host-array mirror (lu.Factorized)."""
+    global _default
+    if _default is None:
+        _default = Context()
+    return _default
 
 
 class Context:

@@ -30,6 +30,7 @@ PREC_FP32 = 1
 PREC_BF16X6 = 2
 PREC_BF16 = 3
 PIVOT_TOURNAMENT = 0x100
+LU_P, LU_L, LU_U, LU_PL = 0, 1, 2, 3
 
 KCLASS = {"panel": 0, "laswp": 1, "trsm": 2, "gemm": 3, "getrs": 4, "residual": 5, "convert": 6,
           "gemm_panel": 7}
@@ -70,6 +71,13 @@ SIGNATURES = [
     ("lair_sgetrf_pivot_device", i32, [vp, i64, i64, vp, i64, vp, i64, i32, i32, P64]),
     ("lair_dgetrs_device", i32, [vp, i64, i64, vp, i64, vp, vp, i64, i32]),
     ("lair_dgetrs_device_noalloc", i32, [vp, i64, i64, vp, i64, vp, vp, i64, i32]),
+    ("lair_dlu_factor", i32, [vp, i64, i64, vp, i64, i64, ctypes.POINTER(vp)]),
+    ("lair_dlu_destroy", i32, [vp]),
+    ("lair_dlu_info", i32, [vp, P64, P64, P64]),
+    ("lair_dlu_factors", i32, [vp, ctypes.POINTER(vp), P64, ctypes.POINTER(vp), ctypes.POINTER(vp)]),
+    ("lair_dlu_solve", i32, [vp, vp, i64, vp, i64, i32]),
+    ("lair_dlu_solve_backward", i32, [vp, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64]),
+    ("lair_dlu_extract", i32, [vp, vp, i32, vp, i64]),
     ("lair_sgetrs_device", i32, [vp, i64, i64, vp, i64, vp, vp, i64, i32]),
     ("lair_dgesv_mixed_device", i32,
      [vp, i64, i64, vp, i64, vp, i64, vp, i64, i32, i32, f64, ctypes.POINTER(IrInfo)]),

@@ -1423,13 +1423,18 @@ lair_status lair_gesv_batched_destroy(lair_batched_plan *plan) {
 
 // ---- adjoint -------------------------------------------------------------
 
+// BBAR = A^-T XBAR by a transposed getrs (on the handle's permutation and
+// diagonal inverses when aux is given, else composed from ipiv), then
+// ABAR = -BBAR X^T.
 static lair_status solve_backward_locked(lair_context *ctx, int n, int nrhs, const double *lu,
                                          int64_t lda, const int64_t *ipiv, const double *x,
                                          int64_t ldx, const double *xbar, int64_t ldxb, double *bbar,
-                                         int64_t ldbb, double *abar, int64_t ldab) {
+                                         int64_t ldbb, double *abar, int64_t ldab,
+                                         const SolveAux<double> *aux = nullptr) {
   hipError_t e = launch_copy_d2d(ctx, n, nrhs, xbar, ldxb, bbar, ldbb);
   if (e != hipSuccess) return hip_status(e, "copy");
-  lair_status st = getrs_impl<double>(ctx, n, nrhs, lu, lda, ipiv, bbar, ldbb, 1);
+  lair_status st = aux ? getrs_aux<double>(ctx, n, nrhs, lu, lda, *aux, bbar, ldbb, 1)
+                       : getrs_impl<double>(ctx, n, nrhs, lu, lda, ipiv, bbar, ldbb, 1);
   if (st != LAIR_OK) return st;
   // X^T (nrhs x n) so that ABAR = -BBAR * X^T is a plain row-major product.
   LAIR_HIP(ensure(ctx->ws_rhs, sizeof(double) * (size_t)n * nrhs));
@@ -1486,6 +1491,160 @@ lair_status lair_dsolve_backward(int64_t n, int64_t nrhs, const double *lu, cons
   LAIR_HIP(hipMemcpyAsync(abar, d_ab, sizeof(double) * nn, hipMemcpyDeviceToHost, ctx->stream));
   return sync_status(ctx);
 }
+
+// ---- device-resident factors ----------------------------------------------
+
+// lu::Factorized (src/decomposition/lu.rs:12-20) kept on the device: the
+// packed factors, the pivots, the row permutation the factorization built and
+// (square systems, built at the first solve) the inverses of the diagonal
+// blocks of L and U, so that every later solve and adjoint reuses them.
+struct lair_lu {
+  int device = 0;
+  int64_t m = 0, n = 0, ld = 0;
+  double *lu = nullptr;
+  int64_t *ipiv = nullptr;
+  int *perm = nullptr;
+  double *linv = nullptr, *uinv = nullptr;
+  int64_t singular = -1;
+};
+
+static void lu_free(lair_lu *h) {
+  for (void *p : {(void *)h->lu, (void *)h->ipiv, (void *)h->perm, (void *)h->linv, (void *)h->uinv})
+    if (p) hipFree(p);
+  delete h;
+}
+
+// The handle's SolveAux; the diagonal inverses are built on first use.
+static lair_status lu_aux(lair_context *ctx, lair_lu *h, SolveAux<double> *aux) {
+  constexpr int BS = TrsvBlock<double>::BS;
+  if (!h->linv) {
+    const size_t bytes = sizeof(double) * (size_t)((h->n + BS - 1) / BS) * BS * BS;
+    LAIR_HIP(hipMalloc(&h->linv, bytes));
+    LAIR_HIP(hipMalloc(&h->uinv, bytes));
+    hipError_t e = launch_trtri_diag<double>(ctx, h->lu, h->ld, (int)h->n, h->linv, h->uinv);
+    if (e != hipSuccess) return hip_status(e, "trtri");
+  }
+  aux->perm = h->perm;
+  aux->linv = h->linv;
+  aux->uinv = h->uinv;
+  return LAIR_OK;
+}
+
+extern "C" {
+
+lair_status lair_dlu_factor(lair_context *ctx, int64_t m, int64_t n, const double *a, int64_t lda, int64_t nb,
+                            lair_lu **out) {
+  if (!ctx) return -1;
+  if (m < 1) return -2;
+  if (n < 1) return -3;
+  if (!a) return -4;
+  if (lda < n) return -5;
+  if (!out) return -7;
+  if (m > INT32_MAX || n > INT32_MAX) return LAIR_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  std::unique_ptr<lair_lu, void (*)(lair_lu *)> h(new lair_lu(), lu_free);
+  h->device = ctx->device;
+  h->m = m;
+  h->n = n;
+  h->ld = pad_ld<double>(n);
+  LAIR_HIP(hipMalloc(&h->lu, sizeof(double) * (size_t)m * h->ld));
+  LAIR_HIP(hipMalloc(&h->ipiv, sizeof(int64_t) * (size_t)std::min(m, n)));
+  LAIR_HIP(hipMalloc(&h->perm, sizeof(int) * (size_t)m));
+  hipError_t e = launch_copy_d2d(ctx, (int)m, (int)n, a, lda, h->lu, h->ld);
+  if (e != hipSuccess) return hip_status(e, "copy");
+  if (nb < 0) {
+    lair_status st = tune_nb<double>(ctx, m, n, h->lu, h->ld, kGemmF64, &nb);
+    if (st != LAIR_OK) return st;
+  }
+  lair_status st = getrf_impl<double>(ctx, m, n, h->lu, h->ld, h->ipiv, nb, kGemmF64, &h->singular, h->perm);
+  if (st != LAIR_OK) return st;
+  *out = h.release();
+  return LAIR_OK;
+}
+
+lair_status lair_dlu_destroy(lair_lu *h) {
+  if (!h) return -1;
+  hipDeviceSynchronize();
+  lu_free(h);
+  return LAIR_OK;
+}
+
+lair_status lair_dlu_info(const lair_lu *h, int64_t *m, int64_t *n, int64_t *singular) {
+  if (!h) return -1;
+  if (m) *m = h->m;
+  if (n) *n = h->n;
+  if (singular) *singular = h->singular;
+  return LAIR_OK;
+}
+
+lair_status lair_dlu_factors(const lair_lu *h, const double **lu, int64_t *ld, const int64_t **ipiv,
+                             const int32_t **perm) {
+  if (!h) return -1;
+  if (lu) *lu = h->lu;
+  if (ld) *ld = h->ld;
+  if (ipiv) *ipiv = h->ipiv;
+  if (perm) *perm = h->perm;
+  return LAIR_OK;
+}
+
+lair_status lair_dlu_solve(lair_context *ctx, lair_lu *h, int64_t nrhs, double *b, int64_t ldb,
+                           int32_t trans) {
+  if (!ctx) return -1;
+  if (!h) return -2;
+  if (nrhs < 0) return -3;
+  if (nrhs > 0 && (!b || ldb < nrhs)) return -4;
+  if (h->m != h->n) {
+    set_error("solve needs a square factorization");
+    return LAIR_ERR_SHAPE;
+  }
+  if (nrhs == 0) return LAIR_OK;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  const int n = (int)h->n;
+  lair_status st;
+  if (!trans && n <= kExactOrderMax) {
+    st = hip_status(launch_getrs_exact<double>(ctx, n, (int)nrhs, h->lu, h->ld, h->ipiv, b, ldb), "getrs");
+  } else {
+    SolveAux<double> aux;
+    st = lu_aux(ctx, h, &aux);
+    if (st == LAIR_OK) st = getrs_aux<double>(ctx, n, (int)nrhs, h->lu, h->ld, aux, b, ldb, trans);
+  }
+  return st != LAIR_OK ? st : sync_status(ctx);
+}
+
+lair_status lair_dlu_solve_backward(lair_context *ctx, lair_lu *h, int64_t nrhs, const double *x, int64_t ldx,
+                                    const double *xbar, int64_t ldxb, double *bbar, int64_t ldbb,
+                                    double *abar, int64_t ldab) {
+  if (!ctx) return -1;
+  if (!h) return -2;
+  if (nrhs < 0) return -3;
+  if (h->m != h->n) {
+    set_error("solve_backward needs a square factorization");
+    return LAIR_ERR_SHAPE;
+  }
+  if (nrhs == 0) return LAIR_OK;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  SolveAux<double> aux;
+  lair_status st = lu_aux(ctx, h, &aux);
+  if (st != LAIR_OK) return st;
+  return solve_backward_locked(ctx, (int)h->n, (int)nrhs, h->lu, h->ld, h->ipiv, x, ldx, xbar, ldxb, bbar,
+                               ldbb, abar, ldab, &aux);
+}
+
+lair_status lair_dlu_extract(lair_context *ctx, const lair_lu *h, int32_t which, double *out, int64_t ldo) {
+  if (!ctx) return -1;
+  if (!h) return -2;
+  if (which < LAIR_LU_P || which > LAIR_LU_PL) return -3;
+  if (!out) return -4;
+  const int64_t k = std::min(h->m, h->n);
+  const int64_t cols = which == LAIR_LU_P ? h->m : (which == LAIR_LU_U ? h->n : k);
+  if (ldo < cols) return -5;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  hipError_t e = launch_lu_extract(ctx, which, (int)h->m, (int)h->n, h->lu, h->ld, h->perm, out, ldo);
+  if (e != hipSuccess) return hip_status(e, "extract");
+  return sync_status(ctx);
+}
+
+}  // extern "C"
 
 // ---- instrumentation -----------------------------------------------------
 

@@ -1033,6 +1033,38 @@ hipError_t launch_copy_d2d(lair_context *ctx, int rows, int cols, const double *
 }
 
 // y (cols x rows) = x^T (x is rows x cols), 32 x 32 tiles through LDS.
+// P (P[perm[j]][j] = 1, lu.rs:28-39), L (unit lower, lu.rs:42-57), U (upper,
+// lu.rs:60-72) or P*L (row perm[i] holds L's row i, lu.rs:107-153), one
+// output element per thread.
+__global__ void lu_extract_kernel(int which, int m, int n, const double *lu, int64_t lda, const int *perm,
+                                  double *out, int64_t ldo, int rows, int cols) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)rows * cols) return;
+  const int i = (int)(idx / cols), j = (int)(idx % cols);
+  const double l = j < i ? lu[(int64_t)i * lda + j] : (j == i ? 1.0 : 0.0);
+  if (which == LAIR_LU_P) {
+    out[(int64_t)i * ldo + j] = perm[j] == i ? 1.0 : 0.0;
+  } else if (which == LAIR_LU_L) {
+    out[(int64_t)i * ldo + j] = l;
+  } else if (which == LAIR_LU_U) {
+    out[(int64_t)i * ldo + j] = j >= i ? lu[(int64_t)i * lda + j] : 0.0;
+  } else {
+    out[(int64_t)perm[i] * ldo + j] = l;
+  }
+}
+
+hipError_t launch_lu_extract(lair_context *ctx, int which, int m, int n, const double *lu, int64_t lda,
+                             const int *perm, double *out, int64_t ldo) {
+  const int k = m < n ? m : n;
+  const int rows = which == LAIR_LU_U ? k : m;
+  const int cols = which == LAIR_LU_P ? m : (which == LAIR_LU_U ? n : k);
+  const int64_t total = (int64_t)rows * cols;
+  if (total <= 0) return hipSuccess;
+  lu_extract_kernel<<<(unsigned)((total + 255) / 256), 256, 0, ctx->stream>>>(which, m, n, lu, lda, perm, out,
+                                                                               ldo, rows, cols);
+  return hipGetLastError();
+}
+
 __global__ void transpose_d_kernel(int rows, int cols, const double *x, int64_t ldx, double *y,
                                    int64_t ldy) {
   __shared__ double tile[32][33];

@@ -304,6 +304,11 @@ hipError_t launch_copy_s2d(lair_context *ctx, int rows, int cols, const float *x
                            double *y, int64_t ldy);
 hipError_t launch_copy_d2d(lair_context *ctx, int rows, int cols, const double *x, int64_t ldx,
                            double *y, int64_t ldy);
+// Factor extraction from packed LU factors (m x n, lda) and the row
+// permutation: which = LAIR_LU_P (m x m), LAIR_LU_L (m x k), LAIR_LU_U
+// (k x n) or LAIR_LU_PL (m x k), k = min(m, n), into out (ldo).
+hipError_t launch_lu_extract(lair_context *ctx, int which, int m, int n, const double *lu, int64_t lda,
+                             const int *perm, double *out, int64_t ldo);
 // y (cols x rows) = x^T.
 hipError_t launch_transpose_d(lair_context *ctx, int rows, int cols, const double *x, int64_t ldx,
                               double *y, int64_t ldy);
