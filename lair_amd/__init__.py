@@ -8,6 +8,7 @@ lib/liblair_dev.so (include/lair_dev.h):
 * ``laswp(...)``         lapack::laswp      src/lapack/laswp.rs:11-40
 * ``equation.solve``     equation::solve    src/equation.rs:32-60
 * ``lu.Factorized``      decomposition::lu  src/decomposition/lu.rs:12-171
+* ``gesv_mixed``         FP32 / BF16 LU with FP64 iterative refinement (new)
 
 Arrays are numpy views; their strides are passed through unchanged (negative
 strides and column-major views are accepted like ndarray views).  Every call
@@ -20,8 +21,8 @@ import numpy as np
 from . import _capi
 from ._capi import LairError, check
 
-__all__ = ["getrf", "getrs", "laswp", "solve_backward", "InvalidInput", "ShapeError", "ValueError_", "equation", "lu",
-           "Context", "DeviceArray", "BatchedPlan", "LairError"]
+__all__ = ["getrf", "getrs", "laswp", "solve_backward", "gesv_mixed", "InvalidInput", "ShapeError", "ValueError_",
+           "equation", "lu", "Context", "DeviceArray", "BatchedPlan", "LairError"]
 
 
 class InvalidInput(Exception):
@@ -121,6 +122,50 @@ def solve_backward(lu_factors, pivots, x, xbar):
     check(lib.lair_dsolve_backward(n, x2.shape[1], _ptr(lu_c), piv.ctypes.data_as(_capi.P64), _ptr(x2),
                                    _ptr(xb2), _ptr(bbar), _ptr(abar)), "solve_backward")
     return bbar.reshape(np.shape(xbar)), abar
+
+
+_PRECISIONS = {"fp64": _capi.PREC_FP64, "fp32": _capi.PREC_FP32, "bf16x6": _capi.PREC_BF16X6,
+               "bf16": _capi.PREC_BF16}
+
+
+def gesv_mixed(a, b, precision="fp32", max_iter=30, tol=0.0, tournament=False):
+    """x = a^-1 b with the LU in a lower precision ("fp32", "bf16x6" = six
+    BF16 products per FP32 product, "bf16", or "fp64") and FP64 iterative
+    refinement until every column's backward error
+    ||b - a x||inf / (||a||inf ||x||inf) is at most tol (0: n * eps64).  A
+    solve that does not converge within max_iter is re-done with the FP64 LU.
+
+    b is n or n x nrhs.  Returns (x, info) with info a dict of iterations,
+    converged, fell_back, precision and backward_error.  New entry point: the
+    reference solves only in the matrix's own precision (src/equation.rs:32-60)."""
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    if a.ndim != 2 or a.shape[0] != a.shape[1] or b.shape[0] != a.shape[0] or b.ndim not in (1, 2):
+        raise ShapeError("gesv_mixed: a must be n x n and b n or n x nrhs")
+    if precision not in _PRECISIONS:
+        raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}")
+    n = a.shape[0]
+    b2 = np.ascontiguousarray(b.reshape(n, -1))
+    nrhs = b2.shape[1]
+    ctx = _default_context()
+    da, db, dx = DeviceArray.from_host(a), DeviceArray.from_host(b2), DeviceArray(n, nrhs)
+    info = _capi.IrInfo()
+    prec = _PRECISIONS[precision] | (_capi.PIVOT_TOURNAMENT if tournament else 0)
+    try:
+        st = ctx.lib.lair_dgesv_mixed_device(ctx.h, n, nrhs, da.ptr, n, db.ptr, nrhs, dx.ptr, nrhs, prec,
+                                             max_iter, tol, ctypes.byref(info))
+        if st == _capi.LAIR_ERR_SINGULAR:
+            raise ValueError_("`a` is a singular matrix")
+        if st != _capi.LAIR_ERR_NOCONV:
+            check(st, "gesv_mixed")
+        x = dx.download()
+    finally:
+        da.free()
+        db.free()
+        dx.free()
+    out = {"iterations": info.iterations, "converged": bool(info.converged), "fell_back": bool(info.fell_back),
+           "precision": info.precision, "backward_error": info.backward_error}
+    return x.reshape(b.shape), out
 
 
 class BatchedPlan:

@@ -162,6 +162,49 @@ def test_pivots_match_oracle(shape):
     assert np.max(np.abs(g - ref)) <= 1e-8 * max(1.0, np.abs(ref).max())
 
 
+@pytest.mark.parametrize("shape", [(17, 17), (129, 129), (300, 200), (200, 300), (1025, 1025), (2049, 1500)])
+@pytest.mark.parametrize("layout", ["F", "neg"])
+def test_pivots_match_col_major_oracle(shape, layout):
+    """Non-standard layouts dispatch to getrf_col_major (getrf.rs:128-213,
+    the dispatch at getrf.rs:17-26): seeded Fortran-order and negative-stride
+    views against the oracle's col-major restatement on the same view."""
+    m, n = shape
+    a = inputs.gaussian_matrix(3000 + m * 5 + n, m, n)
+
+    def view(x):
+        if layout == "F":
+            return np.asfortranarray(x)
+        return np.ascontiguousarray(x[::-1, ::-1])[::-1, ::-1]
+
+    ref = view(a)
+    assert not O.is_standard_layout(m, n, ref.strides[0] // 8, ref.strides[1] // 8)
+    piv_o, sing_o = O.getrf(ref, nthreads=8)
+    g = view(a)
+    piv_g, sing_g = L.getrf(g)
+    assert sing_g == sing_o
+    np.testing.assert_array_equal(piv_g, piv_o)
+    assert lu_backward_error(a, np.array(g), piv_g) <= 8 * max(m, n) * EPS
+    assert np.max(np.abs(g - ref)) <= 1e-8 * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (16, 16), (33, 33), (64, 64), (65, 65), (129, 129), (257, 257),
+                                   (300, 200), (200, 300), (513, 513)])
+def test_sgetrf_pivots_match_oracle(shape):
+    """FP32 dense LU (getrf.rs:46-120 for f32) through lair_sgetrf against the
+    oracle's f32 restatement: pivots bit-exact; factors within n * eps32."""
+    m, n = shape
+    a = inputs.gaussian_matrix(4000 + m * 3 + n, m, n).astype(np.float32)
+    ref = a.copy()
+    piv_o, sing_o = O.sgetrf(ref, nthreads=8)
+    g = a.copy()
+    piv_g, sing_g = L.getrf(g)
+    assert sing_g == sing_o
+    np.testing.assert_array_equal(piv_g, piv_o)
+    eps32 = float(np.finfo(np.float32).eps)
+    assert lu_backward_error(a.astype(np.float64), g.astype(np.float64), piv_g) <= 8 * max(m, n) * eps32
+    assert np.max(np.abs(g - ref)) <= 64 * max(m, n) * eps32 * max(1.0, float(np.abs(ref).max()))
+
+
 def test_c1_fixture_n512():
     fx = json.load(open(os.path.join(HERE, "golden", "c1_n512.json")))
     a = inputs.gaussian_matrix(fx["seed_a"], fx["n"], fx["n"])

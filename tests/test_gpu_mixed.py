@@ -112,3 +112,21 @@ def test_profiling_counts_kernels(ctx):
                                       ctypes.byref(fl), None), "read")
     # FP64 resident leaves are 64 columns wide: two per 128-column panel.
     assert cnt.value == n // 64
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x6", "fp64"])
+def test_python_gesv_mixed(precision):
+    n = 700
+    a = kappa_matrix(77, n)
+    b = inputs.gaussian_matrix(78, n, 3)
+    x, info = L.gesv_mixed(a, b, precision=precision)
+    assert info["converged"] and not info["fell_back"]
+    assert host_berr(a, x, b) <= n * EPS
+    x1, info1 = L.gesv_mixed(a, b[:, 0], precision=precision)
+    assert x1.shape == (n,) and host_berr(a, x1, b[:, :1]) <= n * EPS
+    with pytest.raises(L.ShapeError):
+        L.gesv_mixed(a, b[:5])
+    sing = a.copy()
+    sing[:, 3] = 0.0
+    with pytest.raises(L.ValueError_):
+        L.gesv_mixed(sing, b, precision="fp64")
