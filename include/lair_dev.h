@@ -124,6 +124,28 @@ lair_status lair_dlaswp(int64_t ncols, double *a, int64_t rs, int64_t cs, int64_
 lair_status lair_dsolve(int64_t m, int64_t n, const double *a, int64_t rs, int64_t cs,
                         int64_t blen, const double *b, double *x);
 
+/* Complex<f64> (z) and Complex<f32> (c) forms of getrf, getrs and solve: the
+ * reference's generic A: Scalar instances for Complex<T> (src/scalar.rs:
+ * 310-330), with interleaved (re, im) storage as num-complex 0.4 lays out
+ * Complex<T>; strides count complex elements.  Pivoting picks the first
+ * maximum of |re| + |im| (src/blas/iamax.rs:14); the pivot reciprocal, the
+ * products and getrs's division are num-complex's (getrf.rs:76, getrs.rs:34).
+ * getrs has no transposed form, like the reference's.  Limits: m <= 32768
+ * rows for zgetrf (65536 for cgetrf), n <= 16384 for getrs; beyond them
+ * LAIR_ERR_UNSUPPORTED. */
+lair_status lair_zgetrf(int64_t m, int64_t n, double *a, int64_t rs, int64_t cs, int64_t *ipiv,
+                        int64_t *singular);
+lair_status lair_cgetrf(int64_t m, int64_t n, float *a, int64_t rs, int64_t cs, int64_t *ipiv,
+                        int64_t *singular);
+lair_status lair_zgetrs(int64_t n, int64_t nrhs, const double *lu, int64_t lrs, int64_t lcs,
+                        const int64_t *ipiv, double *b, int64_t brs, int64_t bcs);
+lair_status lair_cgetrs(int64_t n, int64_t nrhs, const float *lu, int64_t lrs, int64_t lcs,
+                        const int64_t *ipiv, float *b, int64_t brs, int64_t bcs);
+lair_status lair_zsolve(int64_t m, int64_t n, const double *a, int64_t rs, int64_t cs,
+                        int64_t blen, const double *b, double *x);
+lair_status lair_csolve(int64_t m, int64_t n, const float *a, int64_t rs, int64_t cs,
+                        int64_t blen, const float *b, float *x);
+
 /* ----------------------------------------------- device-resident calls --- */
 
 /* getrf on a device-resident row-major matrix with leading dimension lda.
@@ -158,6 +180,18 @@ lair_status lair_dgetrs_device(lair_context *ctx, int64_t n, int64_t nrhs, const
 lair_status lair_sgetrs_device(lair_context *ctx, int64_t n, int64_t nrhs, const float *lu,
                                int64_t lda, const int64_t *ipiv, float *b, int64_t ldb,
                                int32_t trans);
+
+/* Complex device forms (interleaved storage, lda/ldb in complex elements,
+ * pointers aligned to the complex element).  nb: panel width, 0 or < 0 the
+ * default (64, the widest complex panel). */
+lair_status lair_zgetrf_device(lair_context *ctx, int64_t m, int64_t n, double *a, int64_t lda,
+                               int64_t *ipiv, int64_t nb, int64_t *singular);
+lair_status lair_cgetrf_device(lair_context *ctx, int64_t m, int64_t n, float *a, int64_t lda,
+                               int64_t *ipiv, int64_t nb, int64_t *singular);
+lair_status lair_zgetrs_device(lair_context *ctx, int64_t n, int64_t nrhs, const double *lu,
+                               int64_t lda, const int64_t *ipiv, double *b, int64_t ldb);
+lair_status lair_cgetrs_device(lair_context *ctx, int64_t n, int64_t nrhs, const float *lu,
+                               int64_t lda, const int64_t *ipiv, float *b, int64_t ldb);
 
 /* lair_dgetrs_device without any device allocation: no counterpart in the
  * reference.  For callers that cannot allocate (code under stream capture,

@@ -46,7 +46,8 @@ def _ptr(a):
 
 
 def getrf(a):
-    """In-place LU with partial pivoting of a 2-D float64/float32 view.
+    """In-place LU with partial pivoting of a 2-D float64, float32, complex128
+    or complex64 view (complex pivots: first maximum of |re| + |im|).
 
     Returns ``(pivots, singular)``: int64 pivots of length min(m, n) (row i was
     interchanged with row pivots[i]) and the last zero-pivot column or None,
@@ -58,12 +59,11 @@ def getrf(a):
     rs, cs = _elem_strides(a)
     piv = np.zeros(min(m, n), np.int64)
     sing = ctypes.c_int64(-1)
-    if a.dtype == np.float64:
-        fn = lib.lair_dgetrf
-    elif a.dtype == np.float32:
-        fn = lib.lair_sgetrf
-    else:
-        raise TypeError("getrf supports float64 and float32")
+    fns = {np.dtype(np.float64): lib.lair_dgetrf, np.dtype(np.float32): lib.lair_sgetrf,
+           np.dtype(np.complex128): lib.lair_zgetrf, np.dtype(np.complex64): lib.lair_cgetrf}
+    fn = fns.get(a.dtype)
+    if fn is None:
+        raise TypeError("getrf supports float64, float32, complex128 and complex64")
     check(fn(m, n, _ptr(a), rs, cs, piv.ctypes.data_as(_capi.P64), ctypes.byref(sing)), "getrf")
     return piv, (None if sing.value < 0 else int(sing.value))
 
@@ -81,14 +81,20 @@ def getrs(a, p, b, trans=False):
     n = len(p)
     x = np.array(b, dtype=a.dtype, copy=True)
     x2 = x.reshape(n, -1)
+    lrs, lcs = _elem_strides(a)
+    brs, bcs = _elem_strides(x2)
+    if a.dtype in (np.complex128, np.complex64):
+        if trans:
+            raise ValueError("getrs: no transposed complex solve (the reference has none)")
+        fn = lib.lair_zgetrs if a.dtype == np.complex128 else lib.lair_cgetrs
+        check(fn(n, x2.shape[1], _ptr(a), lrs, lcs, p.ctypes.data_as(_capi.P64), _ptr(x2), brs, bcs), "getrs")
+        return x
     if a.dtype == np.float64:
         fn = lib.lair_dgetrs
     elif a.dtype == np.float32:
         fn = lib.lair_sgetrs
     else:
-        raise TypeError("getrs supports float64 and float32")
-    lrs, lcs = _elem_strides(a)
-    brs, bcs = _elem_strides(x2)
+        raise TypeError("getrs supports float64, float32, complex128 and complex64")
     check(fn(n, x2.shape[1], _ptr(a), lrs, lcs, p.ctypes.data_as(_capi.P64), _ptr(x2), brs, bcs,
              1 if trans else 0), "getrs")
     return x
@@ -207,16 +213,25 @@ class equation:
 
     @staticmethod
     def solve(a, b):
-        """x = a^-1 b; raises ShapeError / ValueError_ like InvalidInput::{Shape, Value}."""
+        """x = a^-1 b; raises ShapeError / ValueError_ like InvalidInput::{Shape, Value}.
+        float64 unless a or b is complex (then complex128, or complex64 when
+        both are complex64)."""
         lib = _capi.load()
-        a = np.asarray(a, np.float64)
-        b = np.ascontiguousarray(b, np.float64)
+        a = np.asarray(a)
+        b = np.asarray(b)
+        if np.iscomplexobj(a) or np.iscomplexobj(b):
+            dt = np.complex64 if (a.dtype == np.complex64 and b.dtype == np.complex64) else np.complex128
+            fn = lib.lair_csolve if dt == np.complex64 else lib.lair_zsolve
+        else:
+            dt, fn = np.float64, lib.lair_dsolve
+        a = np.asarray(a, dt)
+        b = np.ascontiguousarray(b, dt)
         if a.ndim != 2 or b.ndim != 1:
             raise ShapeError("a must be 2-D and b 1-D")
         m, n = a.shape
         rs, cs = _elem_strides(a)
-        x = np.zeros(max(n, 1), np.float64)
-        st = lib.lair_dsolve(m, n, _ptr(a), rs, cs, len(b), _ptr(b), _ptr(x))
+        x = np.zeros(max(n, 1), dt)
+        st = fn(m, n, _ptr(a), rs, cs, len(b), _ptr(b), _ptr(x))
         if st == _capi.LAIR_ERR_SHAPE:
             raise ShapeError(_capi.last_error())
         if st == _capi.LAIR_ERR_SINGULAR:

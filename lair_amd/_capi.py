@@ -66,6 +66,16 @@ SIGNATURES = [
     ("lair_sgetrs", i32, [i64, i64, vp, i64, i64, P64, vp, i64, i64, i32]),
     ("lair_dlaswp", i32, [i64, vp, i64, i64, i64, i64, P64]),
     ("lair_dsolve", i32, [i64, i64, vp, i64, i64, i64, vp, vp]),
+    ("lair_zgetrf", i32, [i64, i64, vp, i64, i64, P64, P64]),
+    ("lair_cgetrf", i32, [i64, i64, vp, i64, i64, P64, P64]),
+    ("lair_zgetrs", i32, [i64, i64, vp, i64, i64, P64, vp, i64, i64]),
+    ("lair_cgetrs", i32, [i64, i64, vp, i64, i64, P64, vp, i64, i64]),
+    ("lair_zsolve", i32, [i64, i64, vp, i64, i64, i64, vp, vp]),
+    ("lair_csolve", i32, [i64, i64, vp, i64, i64, i64, vp, vp]),
+    ("lair_zgetrf_device", i32, [vp, i64, i64, vp, i64, vp, i64, P64]),
+    ("lair_cgetrf_device", i32, [vp, i64, i64, vp, i64, vp, i64, P64]),
+    ("lair_zgetrs_device", i32, [vp, i64, i64, vp, i64, vp, vp, i64]),
+    ("lair_cgetrs_device", i32, [vp, i64, i64, vp, i64, vp, vp, i64]),
     ("lair_dgetrf_device", i32, [vp, i64, i64, vp, i64, vp, i64, P64]),
     ("lair_sgetrf_device", i32, [vp, i64, i64, vp, i64, vp, i64, P64]),
     ("lair_sgetrf_pivot_device", i32, [vp, i64, i64, vp, i64, vp, i64, i32, i32, P64]),

@@ -13,7 +13,8 @@ from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = ["kernels.hip", "rpanel.hip", "solve_kernels.hip", "gemm_kernels.hip", "batched.hip", "driver.hip"]
+SOURCES = ["kernels.hip", "rpanel.hip", "solve_kernels.hip", "gemm_kernels.hip", "complex.hip", "batched.hip",
+           "driver.hip"]
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "liblair_dev.so")
 # Input generator for the benchmark and large tests (links rocSOLVER/rocBLAS;

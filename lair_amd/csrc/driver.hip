@@ -159,6 +159,29 @@ static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a,
   if (nb > 256) nb = 256;
   if (nb < kLeafW) nb = kLeafW;
   nb = (nb / kLeafW) * kLeafW;
+  constexpr bool kComplex = cplx_traits<T>::is;
+  if constexpr (kComplex) {
+    // Complex panels: resident leaves only, at most kComplexMaxNb wide; the
+    // real embedding of U12 for the trailing updates is sized here, before
+    // either stream of the look-ahead uses it.
+    nb = std::min<int64_t>(nb, kComplexMaxNb);
+    if (kmax > 0 && rpanel_leaf_width<T>(ctx, std::max(std::min<int>((int)nb, kmax), kLeafW), m) == 0) {
+      set_error("too many rows for the complex resident panel");
+      return LAIR_ERR_UNSUPPORTED;
+    }
+    typedef typename cplx_traits<T>::real R;
+    const int64_t lde = 2 * (((int64_t)n + 63) / 64 * 64);
+    LAIR_HIP(ensure(ctx->ws_cexp, sizeof(R) * (size_t)(2 * kComplexMaxNb) * (size_t)lde));
+    ctx->cexp_ld = lde;
+  }
+  // C[M x N] -= A[M x K] B[K x N] inside the matrix (leading dimension lda);
+  // col is the column of C's first element.
+  auto gemm_minus = [&](int M, int N, int K, const T *A, const T *B, T *C, int col, int kclass) -> hipError_t {
+    if constexpr (kComplex)
+      return launch_cgemm_minus<T>(ctx, mode, M, N, K, A, lda, B, lda, C, lda, col, kclass);
+    else
+      return launch_gemm_minus<T>(ctx, mode, M, N, K, A, lda, false, B, lda, C, lda, kclass);
+  };
   hipError_t e = launch_fill_i64(ctx, ctx->sing, 1, -1);
   if (e != hipSuccess) return hip_status(e, "fill");
   if (perm) {
@@ -186,7 +209,9 @@ static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a,
     // Resident leaves (rows kept in LDS for the whole leaf).  A leaf narrower
     // than the panel is followed by its interchanges on the panel's other
     // columns, the U rows right of it (trsm) and the update below (gemm).
-    const int lw = tournament ? 0 : rpanel_leaf_width<T>(ctx, kb, m - k);
+    // (A complex panel narrower than the narrowest leaf goes to a leaf of
+    // that width with fewer columns.)
+    const int lw = tournament ? 0 : rpanel_leaf_width<T>(ctx, kComplex ? std::max(kb, kLeafW) : kb, m - k);
     const int pp = parity(k);
     pmap_ok[pp] = lw >= kb;
     if (lw > 0) {
@@ -209,37 +234,39 @@ static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a,
             e2 = launch_trsm_llu<T>(ctx, npv, rest, a + (int64_t)c * lda + c, lda, a + (int64_t)c * lda + c1,
                                     lda);
           if (e2 == hipSuccess && m > c + npv)
-            e2 = launch_gemm_minus<T>(ctx, mode, m - (c + npv), rest, npv, a + (int64_t)(c + npv) * lda + c,
-                                      lda, false, a + (int64_t)c * lda + c1, lda,
-                                      a + (int64_t)(c + npv) * lda + c1, lda, LAIR_KCLASS_GEMM_PANEL);
+            e2 = gemm_minus(m - (c + npv), rest, npv, a + (int64_t)(c + npv) * lda + c, a + (int64_t)c * lda + c1,
+                            a + (int64_t)(c + npv) * lda + c1, c1, LAIR_KCLASS_GEMM_PANEL);
           if (e2 != hipSuccess) return e2;
         }
       }
       return hipSuccess;
     }
-    for (int c = k; c < k + kb; c += kLeafW) {
-      const int w = std::min(kLeafW, k + kb - c);
-      // The leaf kernel also applies its interchanges to the other columns
-      // of the panel and solves the leaf's U rows right of it.
-      hipError_t e2;
-      if constexpr (std::is_same<T, float>::value) {
-        if (tournament && m - c <= kCaluMaxWG * 2048)
-          e2 = launch_panel_calu(ctx, a, lda, c, m, w, ipiv, k + kb, k);
-        else
+    if constexpr (kComplex) {
+      return hipErrorInvalidValue;  // resident leaves are checked on entry
+    } else {
+      for (int c = k; c < k + kb; c += kLeafW) {
+        const int w = std::min(kLeafW, k + kb - c);
+        // The leaf kernel also applies its interchanges to the other columns
+        // of the panel and solves the leaf's U rows right of it.
+        hipError_t e2;
+        if constexpr (std::is_same<T, float>::value) {
+          if (tournament && m - c <= kCaluMaxWG * 2048)
+            e2 = launch_panel_calu(ctx, a, lda, c, m, w, ipiv, k + kb, k);
+          else
+            e2 = launch_panel<T>(ctx, a, lda, c, m, c, w, ipiv, k + kb, k);
+        } else {
           e2 = launch_panel<T>(ctx, a, lda, c, m, c, w, ipiv, k + kb, k);
-      } else {
-        e2 = launch_panel<T>(ctx, a, lda, c, m, c, w, ipiv, k + kb, k);
-      }
-      if (e2 != hipSuccess) return e2;
-      const int rest = k + kb - (c + w);
-      if (rest > 0 && m > c + w) {
-        e2 = launch_gemm_minus<T>(ctx, mode, m - (c + w), rest, w, a + (int64_t)(c + w) * lda + c, lda,
-                                  false, a + (int64_t)c * lda + c + w, lda,
-                                  a + (int64_t)(c + w) * lda + c + w, lda, LAIR_KCLASS_GEMM_PANEL);
+        }
         if (e2 != hipSuccess) return e2;
+        const int rest = k + kb - (c + w);
+        if (rest > 0 && m > c + w) {
+          e2 = gemm_minus(m - (c + w), rest, w, a + (int64_t)(c + w) * lda + c, a + (int64_t)c * lda + c + w,
+                          a + (int64_t)(c + w) * lda + c + w, c + w, LAIR_KCLASS_GEMM_PANEL);
+          if (e2 != hipSuccess) return e2;
+        }
       }
+      return hipSuccess;
     }
-    return hipSuccess;
   };
   // Panel [k, k + kb)'s interchanges on columns [c0, c1) (and perm).
   auto swap_cols = [&](int k, int kb, int c0, int c1, int *pm) -> hipError_t {
@@ -273,9 +300,8 @@ static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a,
     }
     if (e2 != hipSuccess) return e2;
     if (m > k + kb)
-      e2 = launch_gemm_minus<T>(ctx, mode, m - (k + kb), c1 - c0, kb, a + (int64_t)(k + kb) * lda + k,
-                                lda, false, a + (int64_t)k * lda + c0, lda,
-                                a + (int64_t)(k + kb) * lda + c0, lda);
+      e2 = gemm_minus(m - (k + kb), c1 - c0, kb, a + (int64_t)(k + kb) * lda + k, a + (int64_t)k * lda + c0,
+                      a + (int64_t)(k + kb) * lda + c0, c0, LAIR_KCLASS_GEMM);
     return e2;
   };
   // The bulk updates leave the first (tallest) panel's workgroups their CUs:
@@ -582,15 +608,68 @@ static lair_context *default_ctx() {
 
 // ---- host-pointer entry points ------------------------------------------
 
+// Strided host views to and from dense row-major staging.  Complex elements
+// are copied as their two real parts, so a caller's array needs only the
+// alignment of the real type.
 template <typename T>
 static void gather(int64_t m, int64_t n, const T *a, int64_t rs, int64_t cs, T *dst, int64_t ld) {
-  for (int64_t i = 0; i < m; ++i)
-    for (int64_t j = 0; j < n; ++j) dst[i * ld + j] = a[i * rs + j * cs];
+  if constexpr (cplx_traits<T>::is) {
+    typedef typename cplx_traits<T>::real R;
+    const R *ar = reinterpret_cast<const R *>(a);
+    R *dr = reinterpret_cast<R *>(dst);
+    for (int64_t i = 0; i < m; ++i)
+      for (int64_t j = 0; j < n; ++j) {
+        dr[2 * (i * ld + j)] = ar[2 * (i * rs + j * cs)];
+        dr[2 * (i * ld + j) + 1] = ar[2 * (i * rs + j * cs) + 1];
+      }
+  } else {
+    for (int64_t i = 0; i < m; ++i)
+      for (int64_t j = 0; j < n; ++j) dst[i * ld + j] = a[i * rs + j * cs];
+  }
 }
 template <typename T>
 static void scatter(int64_t m, int64_t n, const T *src, int64_t ld, T *a, int64_t rs, int64_t cs) {
-  for (int64_t i = 0; i < m; ++i)
-    for (int64_t j = 0; j < n; ++j) a[i * rs + j * cs] = src[i * ld + j];
+  if constexpr (cplx_traits<T>::is) {
+    typedef typename cplx_traits<T>::real R;
+    const R *sr = reinterpret_cast<const R *>(src);
+    R *ar = reinterpret_cast<R *>(a);
+    for (int64_t i = 0; i < m; ++i)
+      for (int64_t j = 0; j < n; ++j) {
+        ar[2 * (i * rs + j * cs)] = sr[2 * (i * ld + j)];
+        ar[2 * (i * rs + j * cs) + 1] = sr[2 * (i * ld + j) + 1];
+      }
+  } else {
+    for (int64_t i = 0; i < m; ++i)
+      for (int64_t j = 0; j < n; ++j) a[i * rs + j * cs] = src[i * ld + j];
+  }
+}
+
+// Trailing-update arithmetic of the factorization of T: the real type's.
+template <typename T>
+static constexpr int gemm_mode_of() {
+  return sizeof(typename cplx_traits<T>::real) == 8 ? kGemmF64 : kGemmF32;
+}
+
+// Complex getrs (getrs.rs:12-39 for Complex<R>): permutation from the pivots
+// (range-checked on the device), then the substitutions of launch_cgetrs.
+template <typename T>
+static lair_status cgetrs_impl(lair_context *ctx, int64_t n64, int64_t nrhs64, const T *lu, int64_t lda,
+                               const int64_t *ipiv, T *b, int64_t ldb) {
+  if (n64 < 0) return -2;
+  if (nrhs64 < 0) return -3;
+  if (n64 == 0 || nrhs64 == 0) return LAIR_OK;
+  if (lda < n64) return -5;
+  if (ldb < nrhs64) return -8;
+  if (n64 > kComplexGetrsMax || nrhs64 > INT32_MAX) {
+    set_error("complex getrs supports n <= 16384");
+    return LAIR_ERR_UNSUPPORTED;
+  }
+  const int n = (int)n64;
+  LAIR_HIP(ensure(ctx->ws_perm, sizeof(int) * n));
+  int *perm = static_cast<int *>(ctx->ws_perm.ptr);
+  hipError_t e = launch_perm_from_ipiv(ctx, ipiv, n, perm);
+  if (e == hipSuccess) e = launch_cgetrs<T>(ctx, n, (int)nrhs64, lu, lda, perm, b, ldb);
+  return hip_status(e, "getrs");
 }
 
 template <typename T>
@@ -615,8 +694,7 @@ static lair_status host_getrf(int64_t m, int64_t n, T *a, int64_t rs, int64_t cs
   T *da = static_cast<T *>(ctx->ws_tmp.ptr);
   int64_t *dp = static_cast<int64_t *>(ctx->ws_ipiv.ptr);
   LAIR_HIP(hipMemcpyAsync(da, h.data(), sizeof(T) * m * ld, hipMemcpyHostToDevice, ctx->stream));
-  lair_status st = getrf_impl<T>(ctx, m, n, da, ld, dp, 0,
-                                 std::is_same<T, double>::value ? kGemmF64 : kGemmF32, singular);
+  lair_status st = getrf_impl<T>(ctx, m, n, da, ld, dp, 0, gemm_mode_of<T>(), singular);
   if (st != LAIR_OK) return st;
   LAIR_HIP(hipMemcpyAsync(h.data(), da, sizeof(T) * m * ld, hipMemcpyDeviceToHost, ctx->stream));
   LAIR_HIP(hipMemcpyAsync(ipiv, dp, sizeof(int64_t) * kmin, hipMemcpyDeviceToHost, ctx->stream));
@@ -655,7 +733,11 @@ static lair_status host_getrs(int64_t n, int64_t nrhs, const T *lu, int64_t lrs,
   LAIR_HIP(hipMemcpyAsync(da, h.data(), sizeof(T) * n * ld, hipMemcpyHostToDevice, ctx->stream));
   LAIR_HIP(hipMemcpyAsync(db, hb.data(), sizeof(T) * n * nrhs, hipMemcpyHostToDevice, ctx->stream));
   LAIR_HIP(hipMemcpyAsync(dp, ipiv, sizeof(int64_t) * n, hipMemcpyHostToDevice, ctx->stream));
-  lair_status st = getrs_impl<T>(ctx, n, nrhs, da, ld, dp, db, nrhs, trans);
+  lair_status st;
+  if constexpr (cplx_traits<T>::is)
+    st = trans ? LAIR_ERR_UNSUPPORTED : cgetrs_impl<T>(ctx, n, nrhs, da, ld, dp, db, nrhs);
+  else
+    st = getrs_impl<T>(ctx, n, nrhs, da, ld, dp, db, nrhs, trans);
   if (st != LAIR_OK) return st;
   LAIR_HIP(hipMemcpyAsync(hb.data(), db, sizeof(T) * n * nrhs, hipMemcpyDeviceToHost, ctx->stream));
   st = sync_status(ctx);
@@ -664,6 +746,60 @@ static lair_status host_getrs(int64_t n, int64_t nrhs, const T *lu, int64_t lrs,
   return LAIR_OK;
 }
 
+
+// equation::solve (src/equation.rs:32-60) for Complex<R>: the same checks
+// and errors as lair_dsolve.
+template <typename T>
+static lair_status host_csolve(int64_t m, int64_t n, const T *a, int64_t rs, int64_t cs, int64_t blen,
+                               const T *b, T *x) {
+  if (m < 0 || n < 0 || blen < 0) return -1;
+  if (m != n) {
+    set_error("input matrix is not square");
+    return LAIR_ERR_SHAPE;
+  }
+  if (blen != m) {
+    set_error("The number of elements in `b` must be the same as the number of rows in `a`");
+    return LAIR_ERR_SHAPE;
+  }
+  if (n == 0) return LAIR_OK;
+  std::vector<T> w((size_t)(n * n));
+  gather(n, n, a, rs, cs, w.data(), n);
+  std::vector<int64_t> piv((size_t)n);
+  int64_t sing = -1;
+  lair_status st = host_getrf<T>(n, n, w.data(), n, 1, piv.data(), &sing);
+  if (st != LAIR_OK) return st;
+  if (sing >= 0) {
+    set_error("`a` is a singular matrix");
+    return LAIR_ERR_SINGULAR;
+  }
+  std::vector<T> xb((size_t)n);
+  gather(n, 1, b, 1, 1, xb.data(), 1);
+  st = host_getrs<T>(n, 1, w.data(), n, 1, piv.data(), xb.data(), 1, 1, 0);
+  if (st != LAIR_OK) return st;
+  scatter(n, 1, xb.data(), 1, x, 1, 1);
+  return LAIR_OK;
+}
+
+// Complex device entry points: interleaved (re, im) storage, leading
+// dimensions in complex elements, pointers aligned to the complex element.
+template <typename T>
+static lair_status cgetrf_device(lair_context *ctx, int64_t m, int64_t n, void *a, int64_t lda, int64_t *ipiv,
+                                 int64_t nb, int64_t *singular) {
+  if (!ctx) return -1;
+  if (a && reinterpret_cast<uintptr_t>(a) % sizeof(T)) return -4;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return getrf_impl<T>(ctx, m, n, static_cast<T *>(a), lda, ipiv, nb < 0 ? 0 : nb, gemm_mode_of<T>(), singular);
+}
+template <typename T>
+static lair_status cgetrs_device(lair_context *ctx, int64_t n, int64_t nrhs, const void *lu, int64_t lda,
+                                 const int64_t *ipiv, void *b, int64_t ldb) {
+  if (!ctx) return -1;
+  if ((lu && reinterpret_cast<uintptr_t>(lu) % sizeof(T)) || (b && reinterpret_cast<uintptr_t>(b) % sizeof(T)))
+    return -4;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  lair_status st = cgetrs_impl<T>(ctx, n, nrhs, static_cast<const T *>(lu), lda, ipiv, static_cast<T *>(b), ldb);
+  return st != LAIR_OK ? st : sync_status(ctx);
+}
 
 extern "C" {
 
@@ -682,7 +818,7 @@ lair_status lair_context_create(lair_context **out) {
   LAIR_HIP(hipMalloc(&ctx->drec, sizeof(unsigned long long) * 2 * 64));
   LAIR_HIP(hipMalloc(&ctx->crec, sizeof(unsigned long long) * kCaluWords));
   LAIR_HIP(hipMalloc(&ctx->err, sizeof(int32_t)));
-  LAIR_HIP(hipMalloc(&ctx->rp_hdr, sizeof(unsigned long long) * 2 * kMaxPanelWG * kRpHdrWords));
+  LAIR_HIP(hipMalloc(&ctx->rp_hdr, sizeof(unsigned long long) * kRpHdrTotal));
   LAIR_HIP(hipMalloc(&ctx->rp_rowd, sizeof(unsigned long long) * 2 * kMaxPanelWG * kRpRowWords));
   LAIR_HIP(hipMalloc(&ctx->sing, sizeof(int64_t)));
   LAIR_HIP(hipMalloc(&ctx->swap_rows, sizeof(int) * kMaxSwapRows));
@@ -711,8 +847,7 @@ lair_status lair_context_create(lair_context **out) {
   LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->rec), 4 * kMaxPanelWG * 64, 0u));
   LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->drec), 4 * 64, 0u));
   LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->crec), 2 * kCaluWords, 0u));
-  LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->rp_hdr),
-                           2 * 2 * kMaxPanelWG * kRpHdrWords, 0u));
+  LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->rp_hdr), 2 * kRpHdrTotal, 0u));
   LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->rp_rowd),
                            2 * 2 * kMaxPanelWG * kRpRowWords, 0u));
   ctx->seq = 1;
@@ -861,6 +996,23 @@ lair_status lair_sgetrs_device(lair_context *ctx, int64_t n, int64_t nrhs, const
   return st != LAIR_OK ? st : sync_status(ctx);
 }
 
+lair_status lair_zgetrf_device(lair_context *ctx, int64_t m, int64_t n, double *a, int64_t lda, int64_t *ipiv,
+                               int64_t nb, int64_t *singular) {
+  return cgetrf_device<zcplx>(ctx, m, n, a, lda, ipiv, nb, singular);
+}
+lair_status lair_cgetrf_device(lair_context *ctx, int64_t m, int64_t n, float *a, int64_t lda, int64_t *ipiv,
+                               int64_t nb, int64_t *singular) {
+  return cgetrf_device<ccplx>(ctx, m, n, a, lda, ipiv, nb, singular);
+}
+lair_status lair_zgetrs_device(lair_context *ctx, int64_t n, int64_t nrhs, const double *lu, int64_t lda,
+                               const int64_t *ipiv, double *b, int64_t ldb) {
+  return cgetrs_device<zcplx>(ctx, n, nrhs, lu, lda, ipiv, b, ldb);
+}
+lair_status lair_cgetrs_device(lair_context *ctx, int64_t n, int64_t nrhs, const float *lu, int64_t lda,
+                               const int64_t *ipiv, float *b, int64_t ldb) {
+  return cgetrs_device<ccplx>(ctx, n, nrhs, lu, lda, ipiv, b, ldb);
+}
+
 lair_status lair_dgetrs_device_noalloc(lair_context *ctx, int64_t n, int64_t nrhs, const double *lu,
                                        int64_t lda, const int64_t *ipiv, double *b, int64_t ldb,
                                        int32_t trans) {
@@ -911,6 +1063,25 @@ lair_status lair_dgetrs(int64_t n, int64_t nrhs, const double *lu, int64_t lrs, 
 lair_status lair_sgetrs(int64_t n, int64_t nrhs, const float *lu, int64_t lrs, int64_t lcs,
                         const int64_t *ipiv, float *b, int64_t brs, int64_t bcs, int32_t trans) {
   return host_getrs<float>(n, nrhs, lu, lrs, lcs, ipiv, b, brs, bcs, trans);
+}
+
+lair_status lair_zgetrf(int64_t m, int64_t n, double *a, int64_t rs, int64_t cs, int64_t *ipiv,
+                        int64_t *singular) {
+  return host_getrf<zcplx>(m, n, reinterpret_cast<zcplx *>(a), rs, cs, ipiv, singular);
+}
+lair_status lair_cgetrf(int64_t m, int64_t n, float *a, int64_t rs, int64_t cs, int64_t *ipiv,
+                        int64_t *singular) {
+  return host_getrf<ccplx>(m, n, reinterpret_cast<ccplx *>(a), rs, cs, ipiv, singular);
+}
+lair_status lair_zgetrs(int64_t n, int64_t nrhs, const double *lu, int64_t lrs, int64_t lcs,
+                        const int64_t *ipiv, double *b, int64_t brs, int64_t bcs) {
+  return host_getrs<zcplx>(n, nrhs, reinterpret_cast<const zcplx *>(lu), lrs, lcs, ipiv,
+                           reinterpret_cast<zcplx *>(b), brs, bcs, 0);
+}
+lair_status lair_cgetrs(int64_t n, int64_t nrhs, const float *lu, int64_t lrs, int64_t lcs,
+                        const int64_t *ipiv, float *b, int64_t brs, int64_t bcs) {
+  return host_getrs<ccplx>(n, nrhs, reinterpret_cast<const ccplx *>(lu), lrs, lcs, ipiv,
+                           reinterpret_cast<ccplx *>(b), brs, bcs, 0);
 }
 
 lair_status lair_dlaswp(int64_t ncols, double *a, int64_t rs, int64_t cs, int64_t begin,
@@ -973,6 +1144,17 @@ lair_status lair_dsolve(int64_t m, int64_t n, const double *a, int64_t rs, int64
   }
   std::memcpy(x, b, sizeof(double) * n);
   return lair_dgetrs(n, 1, w.data(), n, 1, piv.data(), x, 1, 1, 0);
+}
+
+lair_status lair_zsolve(int64_t m, int64_t n, const double *a, int64_t rs, int64_t cs, int64_t blen,
+                        const double *b, double *x) {
+  return host_csolve<zcplx>(m, n, reinterpret_cast<const zcplx *>(a), rs, cs, blen,
+                            reinterpret_cast<const zcplx *>(b), reinterpret_cast<zcplx *>(x));
+}
+lair_status lair_csolve(int64_t m, int64_t n, const float *a, int64_t rs, int64_t cs, int64_t blen,
+                        const float *b, float *x) {
+  return host_csolve<ccplx>(m, n, reinterpret_cast<const ccplx *>(a), rs, cs, blen,
+                            reinterpret_cast<const ccplx *>(b), reinterpret_cast<ccplx *>(x));
 }
 
 // ---- device solvers ------------------------------------------------------

@@ -13,8 +13,13 @@
 #include <vector>
 
 #include "../../include/lair_dev.h"
+#include "device_util.h"
 
 namespace lair {
+
+// Complex<f64> and Complex<f32> elements (interleaved re, im).
+typedef cplx<double> zcplx;
+typedef cplx<float> ccplx;
 
 // Width of one cooperative panel leaf (columns factored with exact partial
 // pivoting before the in-panel update).
@@ -44,8 +49,31 @@ constexpr int kRpRowWords = 128;
 // widest panel those kernels take.
 constexpr int kTrsmCW = 16;
 constexpr int kTrsmMaxKb = 128;
-// Tagged header words per workgroup slot of the resident panel's exchange.
+// Tagged header words per workgroup slot of the resident panel's exchange:
+// real elements use 4, complex ones 1 + 2 * (32-bit words per element), in
+// slots of kRpHdrWordsMax.  The two kinds live in separate regions of the
+// header buffer (a real header's first word carries only a 16-bit tag, which
+// a complex word could match): real at 0, complex at kRpHdrComplexOff.
 constexpr int kRpHdrWords = 4;
+constexpr int kRpHdrWordsMax = 16;
+constexpr int kRpHdrComplexOff = 2 * kMaxPanelWG * kRpHdrWords;
+constexpr int kRpHdrTotal = kRpHdrComplexOff + 2 * kMaxPanelWG * kRpHdrWordsMax;  // 64-bit words
+// Widest complex panel (the triangular solve with L11 keeps L11 in LDS),
+// and the largest order of the complex getrs (unknowns of a right-hand side
+// held in registers of one 1024-thread workgroup, 16 per thread).
+constexpr int kComplexMaxNb = 64;
+constexpr int kComplexGetrsMax = 16384;
+
+// Complex trailing update through the real GEMM (complex.hip): the real
+// embedding of U12 (K <= kComplexMaxNb rows) goes to columns [col, col + N)
+// of ctx->ws_cexp, sized by the caller before the factorization starts.
+template <typename T>
+hipError_t launch_cgemm_minus(lair_context *ctx, int mode, int M, int N, int K, const T *A, int64_t lda,
+                              const T *B, int64_t ldb, T *C, int64_t ldc, int col, int kclass);
+// Complex getrs (n <= kComplexGetrsMax): b := U^-1 L^-1 (b permuted by perm).
+template <typename T>
+hipError_t launch_cgetrs(lair_context *ctx, int n, int nrhs, const T *lu, int64_t lda, const int *perm, T *b,
+                         int64_t ldb);
 
 // Diagonal block size of the dataflow triangular solve.
 template <typename T>
@@ -100,7 +128,7 @@ struct lair_context {
   // sets *err (0: every wait gives up at once).
   int32_t wait_limit = 1 << 23;
   // Resident panel (rpanel.hip): header and row slots, tag base, grid cap.
-  unsigned long long *rp_hdr = nullptr;   // [2][kMaxPanelWG][kRpHdrWords]
+  unsigned long long *rp_hdr = nullptr;   // kRpHdrTotal words
   unsigned long long *rp_rowd = nullptr;  // [2][kMaxPanelWG][kRpRowWords]
   uint32_t rp_seq = 1;
   int rp_max_wg = lair::kMaxPanelWG;
@@ -127,6 +155,10 @@ struct lair_context {
   // Grow-only workspaces.
   lair::Buffer ws_a32, ws_lu64, ws_ipiv, ws_x32, ws_r, ws_tmp, ws_rhs;
   lair::Buffer ws_perm, ws_linv, ws_uinv, ws_flags, ws_tmp2;
+  // Complex trailing update: the real embedding of U12, 2 * kComplexMaxNb
+  // rows of cexp_ld real values (two per column of the matrix).
+  lair::Buffer ws_cexp;
+  int64_t cexp_ld = 0;
   // When set, the triangular solves publish their tagged values here instead
   // of in ws_flags (caller-provided memory, zeroed by the caller).
   unsigned long long *xt_override = nullptr;

@@ -39,7 +39,7 @@ struct RpArgs {
   int row0, m, col0, wcols, npiv, P;
   int64_t *ipiv;
   int64_t *sing;
-  unsigned long long *hdr;   // [2][kMaxPanelWG][kRpHdrWords] header words, by column parity
+  unsigned long long *hdr;   // [2][kMaxPanelWG][RpHeader<T>::S] header words, by column parity
   unsigned long long *rowd;  // [2][kMaxPanelWG][kRpRowWords] row words, by column parity
   int *map_rows, *map_src, *map_count;  // row map of the leaf's interchanges (optional)
   uint32_t seq;              // column j's tag is seq + j + 1
@@ -55,9 +55,11 @@ struct RpArgs {
 //   FP64 word 1: tag (32 bits) | low 32 bits of |v|
 //   FP64 word 2: tag (31 bits) | sign of v | high 32 bits of u1
 //   FP64 word 3: tag (32 bits) | low 32 bits of u1
+// S is the stride between two workgroups' headers, in words.
 template <typename T>
 struct RpHeader {
   static constexpr int W = sizeof(T) == 8 ? 4 : 2;
+  static constexpr int S = kRpHdrWords;
   static __device__ __forceinline__ unsigned long long word(int w, uint32_t tag, T v, T u1, int pos) {
     const uint32_t p = pos == INT_MAX ? kRpNone : (uint32_t)pos;
     const uint32_t sgn = v < T(0) ? 1u : 0u;
@@ -86,6 +88,75 @@ struct RpHeader {
     if (sizeof(T) == 4 || w == 2) return (uint32_t)(x >> 33) == (tag & 0x7FFFFFFFu);
     return (uint32_t)(x >> 32) == tag;
   }
+  // Position, |v| (the iamax key), v and u1 of a header whose words all
+  // carry the tag.
+  static __device__ __forceinline__ uint32_t decode(const unsigned long long *w, double &hv, T &hvs, T &hu1) {
+    const uint32_t p = (uint32_t)(w[0] >> 31) & 0x1FFFFu;
+    const uint32_t hi = (uint32_t)w[0] & 0x7FFFFFFFu;
+    uint32_t sgn;
+    if constexpr (sizeof(T) == 8) {
+      hv = __hiloint2double((int)hi, (int)(uint32_t)w[1]);
+      sgn = (uint32_t)(w[2] >> 32) & 1u;
+      hu1 = __hiloint2double((int)(uint32_t)w[2], (int)(uint32_t)w[3]);
+    } else {
+      hv = (double)__uint_as_float(hi);
+      sgn = (uint32_t)(w[1] >> 32) & 1u;
+      hu1 = __uint_as_float((uint32_t)w[1]);
+    }
+    hvs = sgn ? -(T)hv : (T)hv;
+    return p;
+  }
+};
+
+// iamax's key (src/blas/iamax.rs:14: |re| + |im| in the real type) and the
+// pivot reciprocal (getrf.rs:76) of a real or complex element.
+template <typename T>
+__device__ __forceinline__ double rp_key(T x) {
+  if constexpr (cplx_traits<T>::is) {
+    const typename cplx_traits<T>::real k = fabs(x.re) + fabs(x.im);
+    return (double)k;
+  } else {
+    return fabs((double)x);
+  }
+}
+template <typename T>
+__device__ __forceinline__ T rp_recip(T v) {
+  if constexpr (cplx_traits<T>::is)
+    return cplx_recip(v);
+  else
+    return T(1) / v;
+}
+
+// Complex header: the key is recomputed from v by every reader, so the
+// header carries the position and the full values of v and u1, each 32-bit
+// part under a 32-bit tag.
+//   word 0: tag | position; words 1..H: v; words H+1..2H: u1
+template <typename R>
+struct RpHeader<cplx<R>> {
+  typedef cplx<R> T;
+  static constexpr int H = sizeof(T) / 4;
+  static constexpr int W = 1 + 2 * H;
+  static constexpr int S = kRpHdrWordsMax;
+  static __device__ __forceinline__ unsigned long long word(int w, uint32_t tag, T v, T u1, int pos) {
+    const uint32_t p = pos == INT_MAX ? kRpNone : (uint32_t)pos;
+    if (w == 0) return tagged(tag, p);
+    return tagged(tag, w <= H ? word_of<T>(v, w - 1) : word_of<T>(u1, w - 1 - H));
+  }
+  static __device__ __forceinline__ bool ok(int, unsigned long long x, uint32_t tag) {
+    return (uint32_t)(x >> 32) == tag;
+  }
+  static __device__ __forceinline__ uint32_t decode(const unsigned long long *w, double &hv, T &hvs, T &hu1) {
+    uint32_t a[H], b[H];
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      a[k] = (uint32_t)w[1 + k];
+      b[k] = (uint32_t)w[1 + H + k];
+    }
+    hvs = from_words<T>(a);
+    hu1 = from_words<T>(b);
+    hv = rp_key(hvs);
+    return (uint32_t)w[0];
+  }
 };
 
 // Waves: wave 0 communicates (publishes, polls, rebuilds pivot rows) and
@@ -103,7 +174,8 @@ __global__ void __launch_bounds__(64 * (NWK + 1)) rpanel_kernel(RpArgs<T> pa) {
   constexpr int WPE = sizeof(T) / 4;
   constexpr int NI = (NB + 63) / 64;
   constexpr int HW = RpHeader<T>::W;
-  extern __shared__ unsigned char smem_raw[];
+  static_assert(NB * WPE <= kRpRowWords && HW <= RpHeader<T>::S, "exchange record sizes");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   T *s_a = reinterpret_cast<T *>(smem_raw);  // [NB][LDC] the workgroup's rows, by column
   __shared__ T s_wrow[2][NWK][NB];           // worker waves' best rows (columns j-1..)
   __shared__ T s_u[2][NB];                   // pivot rows
@@ -160,7 +232,7 @@ __global__ void __launch_bounds__(64 * (NWK + 1)) rpanel_kernel(RpArgs<T> pa) {
 #pragma unroll
       for (int s = 0; s < RPT; ++s) {
         if (act[s]) {
-          const double x = fabs((double)s_a[j * LDC + tw + s * NW]);
+          const double x = rp_key(s_a[j * LDC + tw + s * NW]);
           if (x > bv || (x == bv && x > 0.0 && cpos[s] < bp)) {
             bv = x;
             bp = cpos[s];
@@ -211,7 +283,7 @@ __global__ void __launch_bounds__(64 * (NWK + 1)) rpanel_kernel(RpArgs<T> pa) {
           }
         }
       }
-      unsigned long long *hs = pa.hdr + ((size_t)par * kMaxPanelWG + g) * kRpHdrWords;
+      unsigned long long *hs = pa.hdr + ((size_t)par * kMaxPanelWG + g) * RpHeader<T>::S;
       if (lane < HW) st_agent(hs + lane, RpHeader<T>::word(lane, tg, cvs, cu1, cp));
       if (cp != INT_MAX) {
         unsigned long long *rs = pa.rowd + ((size_t)par * kMaxPanelWG + g) * kRpRowWords;
@@ -225,7 +297,7 @@ __global__ void __launch_bounds__(64 * (NWK + 1)) rpanel_kernel(RpArgs<T> pa) {
       int hp = INT_MAX;
       T hvs = T(0), hu1 = T(0);
       {
-        const unsigned long long *hb = pa.hdr + (size_t)par * kMaxPanelWG * kRpHdrWords;
+        const unsigned long long *hb = pa.hdr + (size_t)par * kMaxPanelWG * RpHeader<T>::S;
         int spins = 0;
         for (;;) {
           if (spins >= pa.wait_limit) {
@@ -237,22 +309,10 @@ __global__ void __launch_bounds__(64 * (NWK + 1)) rpanel_kernel(RpArgs<T> pa) {
             unsigned long long w[HW];
 #pragma unroll
             for (int k = 0; k < HW; ++k) {
-              w[k] = ld_agent(hb + lane * kRpHdrWords + k);
+              w[k] = ld_agent(hb + lane * RpHeader<T>::S + k);
               ok = ok && RpHeader<T>::ok(k, w[k], tg);
             }
-            const uint32_t p = (uint32_t)(w[0] >> 31) & 0x1FFFFu;
-            const uint32_t hi = (uint32_t)w[0] & 0x7FFFFFFFu;
-            uint32_t sgn;
-            if constexpr (sizeof(T) == 8) {
-              hv = __hiloint2double((int)hi, (int)(uint32_t)w[1]);
-              sgn = (uint32_t)(w[2] >> 32) & 1u;
-              hu1 = __hiloint2double((int)(uint32_t)w[2], (int)(uint32_t)w[3]);
-            } else {
-              hv = (double)__uint_as_float(hi);
-              sgn = (uint32_t)(w[1] >> 32) & 1u;
-              hu1 = __uint_as_float((uint32_t)w[1]);
-            }
-            hvs = sgn ? -(T)hv : (T)hv;
+            const uint32_t p = RpHeader<T>::decode(w, hv, hvs, hu1);
             hp = p == kRpNone ? INT_MAX : (int)p;
             if (hp == INT_MAX) hv = 0.0;
           }
@@ -285,18 +345,9 @@ __global__ void __launch_bounds__(64 * (NWK + 1)) rpanel_kernel(RpArgs<T> pa) {
       }
       if (!none) {
         const int q = __ffsll((long long)__ballot(lane < pa.P && hp == gp)) - 1;
-        T v, u1;
-        if constexpr (sizeof(T) == 8) {
-          v = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(hvs), q),
-                               __builtin_amdgcn_readlane(__double2loint(hvs), q));
-          u1 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(hu1), q),
-                                __builtin_amdgcn_readlane(__double2loint(hu1), q));
-        } else {
-          v = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hvs), q));
-          u1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hu1), q));
-        }
+        const T v = readlane_t(hvs, q), u1 = readlane_t(hu1, q);
         if (lane == 0) {
-          s_recip[par] = T(1) / v;
+          s_recip[par] = rp_recip(v);
           s_u1[par] = u1;
           s_q[par] = q;
           s_u[par][j] = v;
@@ -524,11 +575,15 @@ static hipError_t rp_launch(lair_context *ctx, const RpArgs<T> &pa) {
 template <typename T>
 int rpanel_rows_per_wg(lair_context *ctx, int nbw, int rows) {
   const int maxp = std::min(ctx->rp_max_wg, kMaxPanelWG);
+  // {leaf width, rows per workgroup}: the rows of a workgroup fill about
+  // 130 KB of LDS, and a header or row record fits its exchange slot.
   static const int kF[][2] = {{128, 256}, {64, 512}, {32, 1024}};
   static const int kD[][2] = {{64, 256}, {32, 512}, {16, 1024}};
-  const auto &tab = sizeof(T) == 4 ? kF : kD;
+  static const int kC[][2] = {{64, 256}, {32, 512}, {16, 1024}};  // complex<float>
+  static const int kZ[][2] = {{32, 256}, {16, 512}, {0, 0}};      // complex<double>
+  const auto &tab = sizeof(T) == 4 ? kF : sizeof(T) == 16 ? kZ : cplx_traits<T>::is ? kC : kD;
   for (const auto &e : tab)
-    if (e[0] == nbw && (rows + e[1] - 1) / e[1] <= maxp) return e[1];
+    if (e[0] > 0 && e[0] == nbw && (rows + e[1] - 1) / e[1] <= maxp) return e[1];
   return 0;
 }
 
@@ -552,7 +607,7 @@ hipError_t launch_rpanel(lair_context *ctx, T *a, int64_t lda, int row0, int m, 
   // they could wrap inside this launch.
   if ((ctx->rp_seq & 0xFFFFu) + (uint32_t)nbw + 2u > 0xFFFFu || ctx->rp_seq > 0xF0000000u) {
     hipError_t e = launch_fill_u32(ctx, reinterpret_cast<uint32_t *>(ctx->rp_hdr),
-                                   2 * 2 * kMaxPanelWG * kRpHdrWords, 0u);
+                                   2 * kRpHdrTotal, 0u);
     if (e == hipSuccess)
       e = launch_fill_u32(ctx, reinterpret_cast<uint32_t *>(ctx->rp_rowd),
                           2 * 2 * kMaxPanelWG * kRpRowWords, 0u);
@@ -573,14 +628,21 @@ hipError_t launch_rpanel(lair_context *ctx, T *a, int64_t lda, int row0, int m, 
   pa.map_rows = map_rows;
   pa.map_src = map_src;
   pa.map_count = map_count;
-  pa.hdr = ctx->rp_hdr;
+  pa.hdr = ctx->rp_hdr + (cplx_traits<T>::is ? kRpHdrComplexOff : 0);
   pa.rowd = ctx->rp_rowd;
   pa.seq = ctx->rp_seq;
   pa.err = ctx->err;
   pa.wait_limit = ctx->wait_limit;
   const int slot = prof_begin(ctx, LAIR_KCLASS_PANEL);
   hipError_t e;
-  if constexpr (sizeof(T) == 4) {
+  if constexpr (sizeof(T) == 16) {
+    if (nbw == 32) e = rp_launch<T, 32, 4, 1>(ctx, pa);
+    else e = rp_launch<T, 16, 4, 2>(ctx, pa);
+  } else if constexpr (cplx_traits<T>::is) {
+    if (nbw == 64) e = rp_launch<T, 64, 4, 1>(ctx, pa);
+    else if (nbw == 32) e = rp_launch<T, 32, 4, 2>(ctx, pa);
+    else e = rp_launch<T, 16, 4, 4>(ctx, pa);
+  } else if constexpr (sizeof(T) == 4) {
     if (nbw == 128) e = rp_launch<T, 128, 4, 1>(ctx, pa);
     else if (nbw == 64) e = rp_launch<T, 64, 4, 2>(ctx, pa);
     else e = rp_launch<T, 32, 4, 4>(ctx, pa);
@@ -609,5 +671,13 @@ template hipError_t launch_rpanel<float>(lair_context *, float *, int64_t, int, 
                                          int64_t *, int *, int *, int *);
 template hipError_t launch_rpanel<double>(lair_context *, double *, int64_t, int, int, int, int, int,
                                           int64_t *, int *, int *, int *);
+template int rpanel_grid<zcplx>(lair_context *, int, int);
+template int rpanel_grid<ccplx>(lair_context *, int, int);
+template int rpanel_leaf_width<zcplx>(lair_context *, int, int);
+template int rpanel_leaf_width<ccplx>(lair_context *, int, int);
+template hipError_t launch_rpanel<zcplx>(lair_context *, zcplx *, int64_t, int, int, int, int, int, int64_t *,
+                                         int *, int *, int *);
+template hipError_t launch_rpanel<ccplx>(lair_context *, ccplx *, int64_t, int, int, int, int, int, int64_t *,
+                                         int *, int *, int *);
 
 }  // namespace lair

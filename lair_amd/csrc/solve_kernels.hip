@@ -106,7 +106,7 @@ template <typename T>
 __global__ void __launch_bounds__(256)
     rowgather_kernel(T *a, int64_t lda, const int *rows, const int *src, const int *count, int c0,
                      int n1, int s1, int total) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   T *buf = reinterpret_cast<T *>(smem_raw);  // [nr][32]
   const int t = threadIdx.x, cl = t & 31, rg = t >> 5;
   const int i = blockIdx.x * 32 + cl;
@@ -161,6 +161,7 @@ static hipError_t rowgather(lair_context *ctx, T *a, int64_t lda, int c0, int c1
   const int total = n1 + (c1 - s1);
   if (total > 0) {
     const size_t shm = (size_t)nmax * 32 * sizeof(T);
+    if (shm > 160 * 1024) return hipErrorInvalidValue;
     static bool once = (allow_large_lds(rowgather_kernel<T>), true);
     (void)once;
     rowgather_kernel<T><<<(total + 31) / 32, 256, shm, ctx->stream>>>(a, lda, rows, src, count, c0, n1, s1,
@@ -199,10 +200,18 @@ template hipError_t launch_rowswap_map<double>(lair_context *, double *, int64_t
                                                const int *, const int *, const int *, int, int *);
 template hipError_t launch_rowswap_map<float>(lair_context *, float *, int64_t, int, int, int, int,
                                               const int *, const int *, const int *, int, int *);
+template hipError_t launch_rowswap_map<zcplx>(lair_context *, zcplx *, int64_t, int, int, int, int,
+                                              const int *, const int *, const int *, int, int *);
+template hipError_t launch_rowswap_map<ccplx>(lair_context *, ccplx *, int64_t, int, int, int, int,
+                                              const int *, const int *, const int *, int, int *);
 
 template hipError_t launch_rowswap<double>(lair_context *, double *, int64_t, int, int, int, int,
                                            const int64_t *, int, int, int *);
 template hipError_t launch_rowswap<float>(lair_context *, float *, int64_t, int, int, int, int,
+                                          const int64_t *, int, int, int *);
+template hipError_t launch_rowswap<zcplx>(lair_context *, zcplx *, int64_t, int, int, int, int,
+                                          const int64_t *, int, int, int *);
+template hipError_t launch_rowswap<ccplx>(lair_context *, ccplx *, int64_t, int, int, int, int,
                                           const int64_t *, int, int, int *);
 
 // perm from the full pivot vector (for factors supplied by the caller):
@@ -320,7 +329,7 @@ template <typename T>
 __global__ void __launch_bounds__(256)
     trsm_llu_kernel(int kb, int ncols, const T *__restrict__ L, int64_t ldl, T *__restrict__ B,
                     int64_t ldb) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   T *xs = reinterpret_cast<T *>(smem_raw);  // [kb][64]
   const int t = threadIdx.x, col = t & 63, rg = t >> 6;
   const int c = blockIdx.x * 64 + col;
@@ -365,7 +374,7 @@ __global__ void __launch_bounds__(256)
                       int64_t ldb) {
 #pragma clang fp contract(off)
   constexpr int CW = kTrsmCW, LX = CW + 1;
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   const int LL = kb + 1;
   T *sL = reinterpret_cast<T *>(smem_raw);  // [kb][kb + 1]
   T *sX = sL + (size_t)kb * LL;             // [kb][CW + 1]
@@ -431,7 +440,7 @@ __global__ void __launch_bounds__(256)
 #pragma clang fp contract(off)
   constexpr int CW = kTrsmCW, LX = CW + 1, RG = 256 / CW;
   static_assert(CW == 16, "the diagonal solve maps 4 waves x 4 columns x 16 rows onto 256 threads");
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   T *sL = reinterpret_cast<T *>(smem_raw);             // strictly lower L11, row r at r(r-1)/2
   T *sX = sL + ((size_t)kb * (kb - 1) / 2 + 1);         // [nmax][CW + 1] map rows' old values
   int *s_rows = reinterpret_cast<int *>(sX + (size_t)nmax * LX);  // [nmax]
@@ -494,7 +503,7 @@ __global__ void __launch_bounds__(256)
         lrow[i] = (i < rr && rr < bs) ? sL[(b0 + rr) * (b0 + rr - 1) / 2 + b0 + i] : T(0);
 #pragma unroll
       for (int i = 0; i < 15; ++i) {
-        const T xi = __shfl(x, (l & ~15) | i);
+        const T xi = shfl_t(x, (l & ~15) | i);
         if (i < rr && rr < bs) {
           const T e = xi * lrow[i];
           x = x - e;
@@ -531,6 +540,7 @@ hipError_t launch_swap_trsm(lair_context *ctx, T *a, int64_t lda, int k, int kb,
   if (kb > kTrsmMaxKb || nmax > kMaxSwapRows || nmax < kb) return hipErrorInvalidValue;
   const size_t shm = sizeof(T) * ((size_t)kb * (kb - 1) / 2 + 1 + (size_t)nmax * (kTrsmCW + 1)) +
                      2 * sizeof(int) * (size_t)nmax + 16;
+  if (shm > 160 * 1024) return hipErrorInvalidValue;
   static bool once = (allow_large_lds(swap_trsm_kernel<T>), true);
   (void)once;
   const int slot = prof_begin(ctx, LAIR_KCLASS_TRSM);
@@ -544,6 +554,10 @@ hipError_t launch_swap_trsm(lair_context *ctx, T *a, int64_t lda, int k, int kb,
 template hipError_t launch_swap_trsm<double>(lair_context *, double *, int64_t, int, int, int, int,
                                              const int *, const int *, const int *, int);
 template hipError_t launch_swap_trsm<float>(lair_context *, float *, int64_t, int, int, int, int,
+                                            const int *, const int *, const int *, int);
+template hipError_t launch_swap_trsm<zcplx>(lair_context *, zcplx *, int64_t, int, int, int, int,
+                                            const int *, const int *, const int *, int);
+template hipError_t launch_swap_trsm<ccplx>(lair_context *, ccplx *, int64_t, int, int, int, int,
                                             const int *, const int *, const int *, int);
 
 hipError_t launch_rowmap(lair_context *ctx, const int64_t *ipiv, int k0, int k1) {
@@ -559,6 +573,7 @@ hipError_t launch_trsm_llu(lair_context *ctx, int kb, int ncols, const T *L, int
   if (kb <= 0 || ncols <= 0) return hipSuccess;
   if (kb <= kTrsmMaxKb) {
     const size_t shm = sizeof(T) * ((size_t)kb * (kb + 1) + (size_t)kb * (kTrsmCW + 1));
+    if (shm > 160 * 1024) return hipErrorInvalidValue;
     static bool once16 = (allow_large_lds(trsm_llu16_kernel<T>), true);
     (void)once16;
     const int slot = prof_begin(ctx, LAIR_KCLASS_TRSM);
@@ -583,6 +598,10 @@ template hipError_t launch_trsm_llu<double>(lair_context *, int, int, const doub
                                             double *, int64_t);
 template hipError_t launch_trsm_llu<float>(lair_context *, int, int, const float *, int64_t, float *,
                                            int64_t);
+template hipError_t launch_trsm_llu<zcplx>(lair_context *, int, int, const zcplx *, int64_t, zcplx *,
+                                           int64_t);
+template hipError_t launch_trsm_llu<ccplx>(lair_context *, int, int, const ccplx *, int64_t, ccplx *,
+                                           int64_t);
 
 // ---------------------------------------------------------- trtri_diag ---
 
@@ -592,7 +611,7 @@ template <typename T, int BS>
 __global__ void __launch_bounds__(BS)
     trtri_diag_kernel(const T *__restrict__ lu, int64_t lda, int n, T *__restrict__ linv,
                       T *__restrict__ uinv) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   T *M = reinterpret_cast<T *>(smem_raw);  // [BS][BS+1]
   T *X = M + BS * (BS + 1);                // [BS][BS+1]
   constexpr int LD = BS + 1;
@@ -691,7 +710,7 @@ __global__ void __launch_bounds__(256) trsv_dataflow_kernel(TrsvArgs<T> p) {
   constexpr int NV = BS * BS / VEC / 256;       // 16-byte loads per thread per tile
   constexpr int LDT = BS + VEC;                 // padded LDS row
   typedef T vec_t __attribute__((ext_vector_type(VEC)));
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   T *tile = reinterpret_cast<T *>(smem_raw);    // [BS][LDT]
   __shared__ T xs[BS][RC];
   __shared__ T ys[BS][RC];

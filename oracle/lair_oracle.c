@@ -264,6 +264,171 @@ void lo_zgetrf_row_major(i64 m, i64 n, double *a, i64 rs, i64 cs, i64 *piv, i64 
   *singular = sing;
 }
 
+/* Complex<f32> row-major LU: lo_zgetrf_row_major in single precision (the
+ * reference's generic getrf_row_major with A = Complex<f32>; iamax's key
+ * |re| + |im| and every product and quotient in f32). */
+void lo_cgetrf_row_major(i64 m, i64 n, float *a, i64 rs, i64 cs, i64 *piv, i64 *singular) {
+  i64 dim_min = m < n ? m : n;
+  i64 sing = -1;
+  for (i64 i = 0; i < dim_min; ++i) piv[i] = i;
+#define CE(r, c) (a + 2 * ((r) * rs + (c) * cs))
+  for (i64 ul = 0; ul < dim_min; ++ul) {
+    i64 max_idx = 0;
+    float max_val = 0.0f;
+    for (i64 i = 0; i < m - ul; ++i) {
+      const float *e = CE(ul + i, ul);
+      float v = fabsf(e[0]) + fabsf(e[1]);
+      if (v > max_val) {
+        max_val = v;
+        max_idx = i;
+      }
+    }
+    if (max_idx != 0) {
+      i64 max_row = max_idx + ul;
+      piv[ul] = max_row;
+      for (i64 k = 0; k < n; ++k) {
+        float *p1 = CE(ul, k), *p2 = CE(max_row, k);
+        float t0 = p1[0], t1 = p1[1];
+        p1[0] = p2[0];
+        p1[1] = p2[1];
+        p2[0] = t0;
+        p2[1] = t1;
+      }
+    }
+    if (max_val == 0.0f) {
+      sing = max_idx + ul;
+      continue;
+    }
+    float pr = CE(ul, ul)[0], pi = CE(ul, ul)[1];
+    float ns = pr * pr + pi * pi;
+    float rr = (1.0f * pr + 0.0f * pi) / ns;
+    float ri = (0.0f * pr - 1.0f * pi) / ns;
+    for (i64 j = ul + 1; j < m; ++j) {
+      float *lj = CE(j, ul);
+      float xr = lj[0] * rr - lj[1] * ri;
+      float xi = lj[0] * ri + lj[1] * rr;
+      lj[0] = xr;
+      lj[1] = xi;
+      for (i64 k = ul + 1; k < n; ++k) {
+        const float *u = CE(ul, k);
+        float er = xr * u[0] - xi * u[1];
+        float ei = xr * u[1] + xi * u[0];
+        float *t = CE(j, k);
+        t[0] -= er;
+        t[1] -= ei;
+      }
+    }
+  }
+#undef CE
+  *singular = sing;
+}
+
+/* src/lapack/getrs.rs:12-39 for Complex<f64> (num-complex 0.4), each of the
+ * nrhs columns of b (n x nrhs, row stride ldb; all strides in complex
+ * elements): x = b; forward laswp; x[i] -= a[i][k] * x[k] for k < i in order
+ * (product (ar xr - ai xi, ar xi + ai xr), then the parts subtracted); back
+ * substitution likewise for k > i in order, then x[i] /= a[i][i] as
+ * DivAssign: re = (xr dr + xi di) / |d|^2, im = (xi dr - xr di) / |d|^2 with
+ * |d|^2 = dr dr + di di. */
+void lo_zgetrs(i64 n, i64 nrhs, const double *lu, i64 rs, i64 cs, const i64 *piv, double *b, i64 ldb) {
+  double *x = (double *)malloc(sizeof(double) * 2 * (n > 0 ? n : 1));
+#define ZA(r, c) (lu + 2 * ((r) * rs + (c) * cs))
+  for (i64 r = 0; r < nrhs; ++r) {
+    for (i64 i = 0; i < n; ++i) {
+      x[2 * i] = b[2 * (i * ldb + r)];
+      x[2 * i + 1] = b[2 * (i * ldb + r) + 1];
+    }
+    for (i64 i = 0; i < n; ++i) {
+      const i64 p = piv[i];
+      if (p != i) {
+        double t0 = x[2 * i], t1 = x[2 * i + 1];
+        x[2 * i] = x[2 * p];
+        x[2 * i + 1] = x[2 * p + 1];
+        x[2 * p] = t0;
+        x[2 * p + 1] = t1;
+      }
+    }
+    for (i64 i = 0; i < n; ++i)
+      for (i64 k = 0; k < i; ++k) {
+        const double *e = ZA(i, k);
+        double pr = e[0] * x[2 * k] - e[1] * x[2 * k + 1];
+        double pi = e[0] * x[2 * k + 1] + e[1] * x[2 * k];
+        x[2 * i] -= pr;
+        x[2 * i + 1] -= pi;
+      }
+    for (i64 i = n - 1; i >= 0; --i) {
+      for (i64 k = i + 1; k < n; ++k) {
+        const double *e = ZA(i, k);
+        double pr = e[0] * x[2 * k] - e[1] * x[2 * k + 1];
+        double pi = e[0] * x[2 * k + 1] + e[1] * x[2 * k];
+        x[2 * i] -= pr;
+        x[2 * i + 1] -= pi;
+      }
+      const double *d = ZA(i, i);
+      double xr = x[2 * i], xi = x[2 * i + 1];
+      double ns = d[0] * d[0] + d[1] * d[1];
+      x[2 * i] = (xr * d[0] + xi * d[1]) / ns;
+      x[2 * i + 1] = (xi * d[0] - xr * d[1]) / ns;
+    }
+    for (i64 i = 0; i < n; ++i) {
+      b[2 * (i * ldb + r)] = x[2 * i];
+      b[2 * (i * ldb + r) + 1] = x[2 * i + 1];
+    }
+  }
+#undef ZA
+  free(x);
+}
+
+/* lo_zgetrs for Complex<f32>. */
+void lo_cgetrs(i64 n, i64 nrhs, const float *lu, i64 rs, i64 cs, const i64 *piv, float *b, i64 ldb) {
+  float *x = (float *)malloc(sizeof(float) * 2 * (n > 0 ? n : 1));
+#define CA(r, c) (lu + 2 * ((r) * rs + (c) * cs))
+  for (i64 r = 0; r < nrhs; ++r) {
+    for (i64 i = 0; i < n; ++i) {
+      x[2 * i] = b[2 * (i * ldb + r)];
+      x[2 * i + 1] = b[2 * (i * ldb + r) + 1];
+    }
+    for (i64 i = 0; i < n; ++i) {
+      const i64 p = piv[i];
+      if (p != i) {
+        float t0 = x[2 * i], t1 = x[2 * i + 1];
+        x[2 * i] = x[2 * p];
+        x[2 * i + 1] = x[2 * p + 1];
+        x[2 * p] = t0;
+        x[2 * p + 1] = t1;
+      }
+    }
+    for (i64 i = 0; i < n; ++i)
+      for (i64 k = 0; k < i; ++k) {
+        const float *e = CA(i, k);
+        float pr = e[0] * x[2 * k] - e[1] * x[2 * k + 1];
+        float pi = e[0] * x[2 * k + 1] + e[1] * x[2 * k];
+        x[2 * i] -= pr;
+        x[2 * i + 1] -= pi;
+      }
+    for (i64 i = n - 1; i >= 0; --i) {
+      for (i64 k = i + 1; k < n; ++k) {
+        const float *e = CA(i, k);
+        float pr = e[0] * x[2 * k] - e[1] * x[2 * k + 1];
+        float pi = e[0] * x[2 * k + 1] + e[1] * x[2 * k];
+        x[2 * i] -= pr;
+        x[2 * i + 1] -= pi;
+      }
+      const float *d = CA(i, i);
+      float xr = x[2 * i], xi = x[2 * i + 1];
+      float ns = d[0] * d[0] + d[1] * d[1];
+      x[2 * i] = (xr * d[0] + xi * d[1]) / ns;
+      x[2 * i + 1] = (xi * d[0] - xr * d[1]) / ns;
+    }
+    for (i64 i = 0; i < n; ++i) {
+      b[2 * (i * ldb + r)] = x[2 * i];
+      b[2 * (i * ldb + r) + 1] = x[2 * i + 1];
+    }
+  }
+#undef CA
+  free(x);
+}
+
 /* ------------------------------------------------------ getrf_col_major --- */
 
 /* src/blas/dot.rs:39-133 -- four split accumulators over blocks of four, summed

@@ -33,6 +33,9 @@ def lib():
         L.lo_dgetrf_row_major.argtypes = [i64, i64, dp, i64, i64, ip, ip, ctypes.c_int]
         L.lo_sgetrf_row_major.argtypes = [i64, i64, fp, i64, i64, ip, ip, ctypes.c_int]
         L.lo_zgetrf_row_major.argtypes = [i64, i64, dp, i64, i64, ip, ip]
+        L.lo_cgetrf_row_major.argtypes = [i64, i64, fp, i64, i64, ip, ip]
+        L.lo_zgetrs.argtypes = [i64, i64, dp, i64, i64, ip, dp, i64]
+        L.lo_cgetrs.argtypes = [i64, i64, fp, i64, i64, ip, fp, i64]
         L.lo_dgetrf_col_major.argtypes = [i64, i64, dp, i64, i64, ip, ip]
         L.lo_dgetrf.argtypes = [i64, i64, dp, i64, i64, ip, ip, ctypes.c_int]
         L.lo_dgetrf_recursive.argtypes = [i64, i64, dp, i64, i64, ip]
@@ -98,6 +101,42 @@ def zgetrf(a):
     sing = np.zeros(1, np.int64)
     lib().lo_zgetrf_row_major(m, n, ctypes.cast(a.ctypes.data, dp), n, 1, _p(piv, ip), _p(sing, ip))
     return piv, (None if sing[0] < 0 else int(sing[0]))
+
+
+def cgetrf(a):
+    """getrf.rs:46-120 for Complex32 (interleaved), standard layout."""
+    assert a.dtype == np.complex64 and a.flags.c_contiguous
+    m, n = a.shape
+    piv = np.zeros(min(m, n), np.int64)
+    sing = np.zeros(1, np.int64)
+    lib().lo_cgetrf_row_major(m, n, ctypes.cast(a.ctypes.data, fp), n, 1, _p(piv, ip), _p(sing, ip))
+    return piv, (None if sing[0] < 0 else int(sing[0]))
+
+
+def zgetrs(lu, piv, b):
+    """src/lapack/getrs.rs:12-39 for Complex64, each column of b (n or n x r)."""
+    lu = np.ascontiguousarray(lu, np.complex128)
+    x = np.array(b, np.complex128, copy=True, order="C")
+    one = x.ndim == 1
+    x2 = x.reshape(x.shape[0], -1)
+    n = lu.shape[0]
+    piv = np.ascontiguousarray(piv, np.int64)
+    lib().lo_zgetrs(n, x2.shape[1], ctypes.cast(lu.ctypes.data, dp), lu.shape[1], 1, _p(piv, ip),
+                    ctypes.cast(x2.ctypes.data, dp), x2.shape[1])
+    return x2.reshape(-1) if one else x2
+
+
+def cgetrs(lu, piv, b):
+    """getrs.rs:12-39 for Complex32."""
+    lu = np.ascontiguousarray(lu, np.complex64)
+    x = np.array(b, np.complex64, copy=True, order="C")
+    one = x.ndim == 1
+    x2 = x.reshape(x.shape[0], -1)
+    n = lu.shape[0]
+    piv = np.ascontiguousarray(piv, np.int64)
+    lib().lo_cgetrs(n, x2.shape[1], ctypes.cast(lu.ctypes.data, fp), lu.shape[1], 1, _p(piv, ip),
+                    ctypes.cast(x2.ctypes.data, fp), x2.shape[1])
+    return x2.reshape(-1) if one else x2
 
 
 def getrf_recursive(a):

@@ -156,3 +156,116 @@ def test_getrs_matches_dense_solve(n, nrhs):
     # LAPACK's dgetrs on the oracle's own factors and pivots.
     xl = sl.lu_solve((lu, piv.astype(np.int32)), b, check_finite=False)
     assert np.abs(x - xl).max() <= 1e-12 * np.abs(x).max()
+
+
+# ---------------------------------------------------------------- complex --
+
+def complex_gaussian(seed, m, n, dtype=np.complex128):
+    re = inputs.gaussian_matrix(seed, m, n)
+    im = inputs.gaussian_matrix(seed + 7919, m, n)
+    return (re + 1j * im).astype(dtype)
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (7, 7), (64, 64), (129, 129), (300, 200), (200, 300), (512, 512)])
+def test_zgetrf_pivots_equal_lapack(shape):
+    """LAPACK's zgetrf picks pivots with izamax, whose key is |re| + |im|
+    (dcabs1), the reference's iamax key (src/blas/iamax.rs:14)."""
+    m, n = shape
+    a = complex_gaussian(2300 + m + n, m, n)
+    w = a.copy()
+    piv, sing = O.zgetrf(w)
+    assert sing is None
+    k = min(m, n)
+    assert np.array_equal(piv, lapack_pivots(a)[:k])
+    perm = O.lu_perm(m, piv)
+    l = np.tril(w[:, :k], -1) + np.eye(m, k)
+    u = np.triu(w[:k, :])
+    assert np.abs(a[perm, :] - l @ u).max() <= 8 * max(m, n) * EPS * np.abs(a).max()
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (7, 7), (33, 33), (64, 40), (40, 64)])
+def test_cgetrf_pivots_equal_lapack(shape):
+    m, n = shape
+    a = complex_gaussian(2400 + m + n, m, n, np.complex64)
+    w = a.copy()
+    piv, sing = O.cgetrf(w)
+    assert sing is None
+    k = min(m, n)
+    assert np.array_equal(piv, lapack_pivots(a)[:k])
+    perm = O.lu_perm(m, piv)
+    l = np.tril(w[:, :k], -1) + np.eye(m, k)
+    u = np.triu(w[:k, :])
+    eps32 = float(np.finfo(np.float32).eps)
+    assert np.abs(a[perm, :] - l @ u).max() <= 8 * max(m, n) * eps32 * np.abs(a).max()
+
+
+def test_cgetrf_square_complex_known_answer():  # src/lapack/getrf.rs:485-502 in f32
+    a = np.array([[1 + 1j, 2 - 1j], [3 + 1j, 4 - 1j]], np.complex64)
+    piv, sing = O.cgetrf(a)
+    assert list(piv) == [1, 1] and sing is None
+    want = np.array([[3 + 1j, 4 - 1j], [0.4 + 0.2j, 0.2 - 1.4j]])
+    assert np.abs(a - want).max() <= 1e-6
+
+
+def _gauss_int(rng, m, n, lo=-3, hi=4):
+    return (rng.integers(lo, hi, size=(m, n)) + 1j * rng.integers(lo, hi, size=(m, n))).astype(np.complex128)
+
+
+def _exact_getrs(lu, piv, b):
+    """Exact integer arithmetic on Gaussian-integer factors whose diagonal is a
+    unit (1, -1, i, -i): x = U^-1 L^-1 P b with no rounding at all."""
+    n = lu.shape[0]
+    L = [[(int(v.real), int(v.imag)) for v in row] for row in lu]
+    x = [(int(v.real), int(v.imag)) for v in b]
+    for i, p in enumerate(piv):
+        x[i], x[p] = x[p], x[i]
+
+    def mul(a, c):
+        return (a[0] * c[0] - a[1] * c[1], a[0] * c[1] + a[1] * c[0])
+
+    for i in range(n):
+        for k in range(i):
+            pr = mul(L[i][k], x[k])
+            x[i] = (x[i][0] - pr[0], x[i][1] - pr[1])
+    for i in range(n - 1, -1, -1):
+        for k in range(i + 1, n):
+            pr = mul(L[i][k], x[k])
+            x[i] = (x[i][0] - pr[0], x[i][1] - pr[1])
+        d = L[i][i]
+        assert d[0] * d[0] + d[1] * d[1] == 1
+        x[i] = mul(x[i], (d[0], -d[1]))  # x / d = x * conj(d) for a unit d
+    return np.array([complex(*v) for v in x])
+
+
+@pytest.mark.parametrize("n,dtype", [(1, np.complex128), (2, np.complex128), (9, np.complex128),
+                                     (24, np.complex128), (1, np.complex64), (5, np.complex64),
+                                     (10, np.complex64)])
+def test_complex_getrs_exact_on_gaussian_integers(n, dtype):
+    """Unit-modulus diagonal and small Gaussian-integer off-diagonal factors:
+    every product, difference and quotient is exact, so the oracle must match
+    exact integer arithmetic bit for bit.  A conjugated product, a wrong
+    division formula, a transposed factor or a skipped interchange fails."""
+    rng = np.random.default_rng(n * 31 + (dtype == np.complex64))
+    lu = np.tril(_gauss_int(rng, n, n, -1, 2), -1) + np.triu(_gauss_int(rng, n, n, -1, 2), 1)
+    units = np.array([1, -1, 1j, -1j])
+    lu[np.arange(n), np.arange(n)] = units[rng.integers(0, 4, n)]
+    piv = np.array([rng.integers(i, n) for i in range(n)], np.int64)
+    b = _gauss_int(rng, n, 1).ravel()
+    want = _exact_getrs(lu, piv, b)
+    limit = 2 ** 23 if dtype == np.complex64 else 2 ** 52
+    assert np.abs(want.real).max() < limit and np.abs(want.imag).max() < limit
+    fn = O.cgetrs if dtype == np.complex64 else O.zgetrs
+    got = fn(lu.astype(dtype), piv, b.astype(dtype))
+    assert np.array_equal(got.astype(np.complex128), want)
+
+
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (10, 3), (257, 4)])
+def test_zgetrs_matches_dense_solve(n, nrhs):
+    a = complex_gaussian(2500 + n, n, n)
+    b = complex_gaussian(2501 + n, n, nrhs)
+    lu = a.copy()
+    piv, _ = O.zgetrf(lu)
+    x = O.zgetrs(lu, piv, b)
+    assert np.abs(x - np.linalg.solve(a, b)).max() <= 1e-9 * np.abs(x).max()
+    xl = sl.lu_solve((lu, piv.astype(np.int32)), b, check_finite=False)
+    assert np.abs(x - xl).max() <= 1e-12 * np.abs(x).max()
