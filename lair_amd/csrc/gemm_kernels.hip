@@ -65,81 +65,160 @@ struct Mfma16<float> {
   static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
+// Raw buffer access: a wave-uniform resource (scalar registers), a 32-bit
+// per-lane byte offset and a wave-uniform byte offset.  Keeps the many
+// elements of a tile at one vector register of address.
+typedef unsigned int u2v __attribute__((ext_vector_type(2)));
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+constexpr int kBufWord3 = 0x00020000;  // raw buffer, 32-bit data format (gfx9)
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, 0x7fffffff, kBufWord3);
+}
+template <typename T>
+__device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <typename T>
+__device__ __forceinline__ void buf_st(T x, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  if constexpr (sizeof(T) == 8)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2v, x), r, voff, soff, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, voff, soff, 0);
+}
+// Two consecutive elements (8 or 16 bytes).
+template <typename T, typename P2>
+__device__ __forceinline__ P2 buf_ld2(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(P2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+  else
+    return __builtin_bit_cast(P2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
 // STORE = false: C -= op(A) B (trailing update); STORE = true: C = -op(A) B
 // (C is only written, e.g. the adjoint's A-gradient).
 //
-// TPW > 1 (capped grids): a workgroup of 256 * TPW threads computes TPW
-// horizontally adjacent tiles at once (one group of four waves each, own LDS
-// stages), so one workgroup per CU has the occupancy of TPW workgroups.
-template <typename T, bool TRANSA, bool STORE, int TPW>
-__global__ void __launch_bounds__(256 * TPW, TPW == 1 ? 2 : 1)
-    gemm_tile_kernel(int M, int N, int K, const T *__restrict__ A, int64_t lda,
-                     const T *__restrict__ B, int64_t ldb, T *__restrict__ C, int64_t ldc) {
+// The accumulators start from C (or zero) and the MFMAs add (-A) B: A is
+// negated as it is staged in LDS.  So C is read at the start of a tile, its
+// latency overlapping the first stages, and each accumulator is stored once at
+// the end with no second copy of C in registers.
+//
+// k order inside a stage of BK = 16: MFMA step s (0..3) takes k = 4 g + s
+// from the lane group g = lane / 16 (the MFMA's k index within the step), so
+// a lane's four k values of one 16 x 16 block are contiguous in LDS: A is
+// staged as [row][k] and B transposed as [col][k], and each block costs one
+// (FP32) or two (FP64) 16-byte LDS reads per lane.  The row pitch P (20
+// floats, 18 doubles) puts the 16 rows that 16 lanes read at once on
+// distinct 16-byte bank groups.
+//
+// Memory access goes through raw buffer resources based at the tile (scalar
+// registers) with one 32-bit lane offset and uniform element offsets, so the
+// 64 C elements of a lane cost no address registers.  Interior tiles with K a
+// multiple of BK run unmasked; the others (MASKED) load from clamped, always
+// valid addresses and select zero outside the matrix.
+constexpr int kBK = 16;
+
+template <typename T, bool TRANSA, bool STORE, bool MASKED>
+__device__ __forceinline__ void gemm_tile(int M, int N, int K, const T *__restrict__ A, int64_t lda,
+                                          const T *__restrict__ B, int64_t ldb, T *__restrict__ C, int64_t ldc,
+                                          int bm, int bn, bool ghost,
+                                          T (*As)[kTileM][kBK + (sizeof(T) == 4 ? 4 : 2)],
+                                          T (*Bs)[kTileN][kBK + (sizeof(T) == 4 ? 4 : 2)]) {
   typedef typename Mfma16<T>::acc_t acc_t;
   typedef typename Mfma16<T>::pair_t d2;
-  constexpr int BK = 16;
-  constexpr int PA = BK + 1;      // padded row of the A stage (16 lanes read 16 rows)
-  constexpr int PB = kTileN + 4;
-  __shared__ T As_all[TPW][2][kTileM][PA];  // As[i][k]
-  __shared__ T Bs_all[TPW][2][BK][PB];      // Bs[k][j]
-  const int h = threadIdx.x >> 8;
-  auto &As = As_all[h];
-  auto &Bs = Bs_all[h];
-  const int t = threadIdx.x & 255, lane = t & 63, wave = t >> 6;
-  const int nbm = (M + kTileM - 1) / kTileM, nbn = (N + kTileN - 1) / kTileN;
-  const int nbnp = (nbn + TPW - 1) / TPW;
-  const int ntile = nbm * nbnp;
-  // A capped grid loops over the tiles (same XCD grouping: gridDim.x is a
-  // multiple of 8 whenever it is below the tile count).  A tile past the
-  // right edge (tn >= nbn) has every load and store masked.
-  for (int id = blockIdx.x; id < ntile; id += gridDim.x) {
-  int tm, tnp;
-  tile_coords(id, nbm, nbnp, tm, tnp);
-  const int tn = tnp * TPW + h;
-  const int bm = tm * kTileM, bn = tn * kTileN;
-  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  typedef T v16 __attribute__((ext_vector_type(16 / sizeof(T))));  // 16 bytes
+  constexpr int BK = kBK;
+  constexpr int VW = 16 / sizeof(T);  // elements per 16-byte access
+  const int t = threadIdx.x & 255, lane = t & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((threadIdx.x & 255) >> 6);
+  const int wm = (wv >> 1) * 64, wn = (wv & 1) * 64;
+
+  // C: this lane's element (i, j, r) is at row i*16 + r*rstep, column j*16
+  // of the wave's quadrant, plus the lane offset.
+  const int lr0 = Mfma16<T>::row(lane, 0), rstep = Mfma16<T>::row(0, 1);
+  const int mrem = M - (bm + wm + lr0), nrem = N - (bn + wn + (lane & 15));
+  const int coff = (lr0 * (int)ldc + (lane & 15)) * (int)sizeof(T);
+  auto csoff = [&](int i, int j, int r) { return ((i * 16 + r * rstep) * (int)ldc + j * 16) * (int)sizeof(T); };
+  const __amdgpu_buffer_rsrc_t crs = buf_rsrc(C + (int64_t)(bm + wm) * ldc + (bn + wn));
+  acc_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (STORE) {
+          acc[i][j][r] = T(0);
+        } else if (!MASKED) {
+          acc[i][j][r] = buf_ld<T>(crs, coff, csoff(i, j, r));
+        } else {
+          // Clamped into the matrix: an element outside it is never stored,
+          // so what it accumulates does not matter.
+          const int gi = min(bm + wm + lr0 + i * 16 + r * rstep, M - 1);
+          const int gj = min(bn + wn + (lane & 15) + j * 16, N - 1);
+          acc[i][j][r] = C[(int64_t)gi * ldc + gj];
+        }
+      }
 
   // Global fetch mapping: A (not transposed): thread -> row t/2, k (t%2)*8..+8;
   // A transposed (stored K x M): thread -> k t/16, rows (t%16)*8..+8;
-  // B: thread -> k t/16, cols (t%16)*8..+8.
+  // B: thread -> column t%128, k (t/128)*8..+8 (coalesced along the row).
   T ra[8], rb[8];
   auto fetch = [&](int k0) {
-    if (!TRANSA) {
-      const int i = t >> 1, kk = (t & 1) * 8;
-      const int gi = bm + i;
+    const int j = t & 127, kb = (t >> 7) * 8;
+    if (!MASKED) {
+      const __amdgpu_buffer_rsrc_t brs = buf_rsrc(B + (int64_t)k0 * ldb + bn);
+      const int bvo = (kb * (int)ldb + j) * (int)sizeof(T);
 #pragma unroll
-      for (int q = 0; q < 8; q += 2) {
-        const int gk = k0 + kk + q;
-        if (gi < M && gk + 1 < K) {
-          const d2 v = *reinterpret_cast<const d2 *>(A + (int64_t)gi * lda + gk);
+      for (int q = 0; q < 8; ++q) rb[q] = buf_ld<T>(brs, bvo, q * (int)ldb * (int)sizeof(T));
+      if (!TRANSA) {
+        const int i = t >> 1, kk = (t & 1) * 8;
+        const __amdgpu_buffer_rsrc_t ars = buf_rsrc(A + (int64_t)bm * lda + k0);
+        const int avo = (i * (int)lda + kk) * (int)sizeof(T);
+#pragma unroll
+        for (int q = 0; q < 8; q += 2) {
+          const d2 v = buf_ld2<T, d2>(ars, avo, q * (int)sizeof(T));
           ra[q] = v[0];
           ra[q + 1] = v[1];
-        } else {
-          ra[q] = (gi < M && gk < K) ? A[(int64_t)gi * lda + gk] : T(0);
-          ra[q + 1] = (gi < M && gk + 1 < K) ? A[(int64_t)gi * lda + gk + 1] : T(0);
         }
+      } else {
+        const int kk = t >> 4, i0 = (t & 15) * 8;
+        const __amdgpu_buffer_rsrc_t ars = buf_rsrc(A + (int64_t)k0 * lda + bm);
+        const int avo = (kk * (int)lda + i0) * (int)sizeof(T);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) ra[q] = buf_ld<T>(ars, avo, q * (int)sizeof(T));
       }
     } else {
-      const int kk = t >> 4, i0 = (t & 15) * 8;
-      const int gk = k0 + kk;
+      // Clamped indices keep every load inside the matrix.  Rows of A and
+      // columns of B outside it only reach accumulators that are never
+      // stored; k beyond K is zeroed in both operands.
+      const int cj = min(bn + j, N - 1);
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const int gi = bm + i0 + q;
-        ra[q] = (gk < K && gi < M) ? A[(int64_t)gk * lda + gi] : T(0);
+        const int gk = k0 + kb + q;
+        const T v = B[(int64_t)min(gk, K - 1) * ldb + cj];
+        rb[q] = gk < K ? v : T(0);
       }
-    }
-    const int kb = t >> 4, j0 = (t & 15) * 8;
-    const int gk = k0 + kb;
+      if (!TRANSA) {
+        const int kk = (t & 1) * 8;
+        const T *ar = A + (int64_t)min(bm + (t >> 1), M - 1) * lda;
 #pragma unroll
-    for (int q = 0; q < 8; q += 2) {
-      const int gj = bn + j0 + q;
-      if (gk < K && gj + 1 < N) {
-        const d2 v = *reinterpret_cast<const d2 *>(B + (int64_t)gk * ldb + gj);
-        rb[q] = v[0];
-        rb[q + 1] = v[1];
+        for (int q = 0; q < 8; ++q) {
+          const int gk = k0 + kk + q;
+          const T v = ar[min(gk, K - 1)];
+          ra[q] = gk < K ? v : T(0);
+        }
       } else {
-        rb[q] = (gk < K && gj < N) ? B[(int64_t)gk * ldb + gj] : T(0);
-        rb[q + 1] = (gk < K && gj + 1 < N) ? B[(int64_t)gk * ldb + gj + 1] : T(0);
+        const int gk = k0 + (t >> 4), i0 = (t & 15) * 8;
+        const T *ak = A + (int64_t)min(gk, K - 1) * lda;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const T v = ak[min(bm + i0 + q, M - 1)];
+          ra[q] = gk < K ? v : T(0);
+        }
       }
     }
   };
@@ -147,115 +226,182 @@ __global__ void __launch_bounds__(256 * TPW, TPW == 1 ? 2 : 1)
     if (!TRANSA) {
       const int i = t >> 1, kk = (t & 1) * 8;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) As[buf][i][kk + q] = ra[q];
+      for (int q = 0; q < 8; q += VW) {
+        v16 v;
+#pragma unroll
+        for (int e = 0; e < VW; ++e) v[e] = -ra[q + e];
+        *reinterpret_cast<v16 *>(&As[buf][i][kk + q]) = v;
+      }
     } else {
       const int kk = t >> 4, i0 = (t & 15) * 8;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) As[buf][i0 + q][kk] = ra[q];
+      for (int q = 0; q < 8; ++q) As[buf][i0 + q][kk] = -ra[q];
     }
-    const int kb = t >> 4, j0 = (t & 15) * 8;
+    const int j = t & 127, kb = (t >> 7) * 8;
 #pragma unroll
-    for (int q = 0; q < 8; q += 2) *reinterpret_cast<d2 *>(&Bs[buf][kb][j0 + q]) = (d2){rb[q], rb[q + 1]};
+    for (int q = 0; q < 8; q += VW) {
+      v16 v;
+#pragma unroll
+      for (int e = 0; e < VW; ++e) v[e] = rb[q + e];
+      *reinterpret_cast<v16 *>(&Bs[buf][j][kb + q]) = v;
+    }
   };
 
-  acc_t acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = (acc_t){T(0), T(0), T(0), T(0)};
-
   const int nstage = (K + BK - 1) / BK;
+  const int kq = 4 * (lane >> 4);  // this lane's first k within a stage
   fetch(0);
   stash(0);
   __syncthreads();
   for (int s = 0; s < nstage; ++s) {
     const int buf = s & 1;
     if (s + 1 < nstage) fetch((s + 1) * BK);
+    // Two column blocks of B at a time, A re-read per pair.
 #pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
-      T av[4], bv[4];
+    for (int jp = 0; jp < 4; jp += 2) {
+      T bq[2][4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) av[i] = As[buf][wm + i * 16 + (lane & 15)][kk + (lane >> 4)];
+      for (int j = 0; j < 2; ++j) {
+        const T *src = &Bs[buf][wn + (jp + j) * 16 + (lane & 15)][kq];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) bv[j] = Bs[buf][kk + (lane >> 4)][wn + j * 16 + (lane & 15)];
+        for (int q = 0; q < 4; q += VW) {
+          const v16 v = *reinterpret_cast<const v16 *>(src + q);
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+          for (int e = 0; e < VW; ++e) bq[j][q + e] = v[e];
+        }
+      }
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-          acc[i][j] = Mfma16<T>::op(av[i], bv[j], acc[i][j]);
+      for (int i = 0; i < 4; ++i) {
+        T aq[4];
+        const T *src = &As[buf][wm + i * 16 + (lane & 15)][kq];
+#pragma unroll
+        for (int q = 0; q < 4; q += VW) {
+          const v16 v = *reinterpret_cast<const v16 *>(src + q);
+#pragma unroll
+          for (int e = 0; e < VW; ++e) aq[q + e] = v[e];
+        }
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) acc[i][jp + j] = Mfma16<T>::op(aq[ks], bq[j][ks], acc[i][jp + j]);
+      }
     }
     if (s + 1 < nstage) stash(buf ^ 1);
     __syncthreads();
   }
 
-  // Epilogue: per 16-row block, all 16 loads of C are issued before any store.
+  // Offsets and bounds are recomputed from opaque copies rather than kept
+  // (64 scalar registers, 64 masks) across the K loop.
+  int ldc_e = (int)ldc, mrem_e = mrem, nrem_e = nrem;
+  asm volatile("" : "+s"(ldc_e));
+  asm volatile("" : "+v"(mrem_e), "+v"(nrem_e));
+  if (ghost) return;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    T cv[4][4];
-    if (!STORE) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int gi = bm + wm + i * 16 + Mfma16<T>::row(lane, r);
-          const int gj = bn + wn + j * 16 + (lane & 15);
-          cv[j][r] = (gi < M && gj < N) ? C[(int64_t)gi * ldc + gj] : T(0);
-        }
-    }
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gi = bm + wm + i * 16 + Mfma16<T>::row(lane, r);
-        const int gj = bn + wn + j * 16 + (lane & 15);
-        if (gi < M && gj < N) C[(int64_t)gi * ldc + gj] = STORE ? -acc[i][j][r] : cv[j][r] - acc[i][j][r];
+        if (!MASKED || (i * 16 + r * rstep < mrem_e && j * 16 < nrem_e))
+          buf_st<T>(acc[i][j][r], crs, coff, ((i * 16 + r * rstep) * ldc_e + j * 16) * (int)sizeof(T));
       }
+}
+
+// TPW > 1 (capped grids): a workgroup of 256 * TPW threads computes TPW
+// horizontally adjacent tiles at once (one group of four waves each, own LDS
+// stages), so one workgroup per CU has the occupancy of TPW workgroups.
+// MASKED = false: every tile lies inside M x N and K is a multiple of kBK; a
+// group whose tile would lie past the right edge computes the last tile of
+// its row again (keeping the workgroup's barriers) and stores nothing.
+template <typename T, bool TRANSA, bool STORE, int TPW, bool MASKED>
+__global__ void __launch_bounds__(256 * TPW, TPW == 1 ? 2 : 1)
+    gemm_tile_kernel(int M, int N, int K, const T *__restrict__ A, int64_t lda,
+                     const T *__restrict__ B, int64_t ldb, T *__restrict__ C, int64_t ldc) {
+  constexpr int P = kBK + (sizeof(T) == 4 ? 4 : 2);
+  __shared__ __align__(16) T As_all[TPW][2][kTileM][P];  // As[i][k] = -A[i][k]
+  __shared__ __align__(16) T Bs_all[TPW][2][kTileN][P];  // Bs[j][k] = B[k][j]
+  const int h = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8);
+  const int nbm = (M + kTileM - 1) / kTileM, nbn = (N + kTileN - 1) / kTileN;
+  const int nbnp = (nbn + TPW - 1) / TPW;
+  const int ntile = nbm * nbnp;
+  // A capped grid loops over the tiles (same XCD grouping: gridDim.x is a
+  // multiple of 8 whenever it is below the tile count).
+  for (int id = blockIdx.x; id < ntile; id += gridDim.x) {
+    int tm, tnp;
+    tile_coords(id, nbm, nbnp, tm, tnp);
+    const int tn = tnp * TPW + h;
+    const bool ghost = tn >= nbn;
+    const int bm = tm * kTileM, bn = (ghost && !MASKED ? nbn - 1 : tn) * kTileN;
+    gemm_tile<T, TRANSA, STORE, MASKED>(M, N, K, A, lda, B, ldb, C, ldc, bm, bn, ghost, As_all[h], Bs_all[h]);
   }
+}
+
+// Splits C into the region of whole tiles (unmasked kernel, launched by
+// `interior`) and the right and bottom strips (masked kernel, one tile per
+// workgroup); K not a multiple of kBK runs masked throughout.  The 32-bit
+// buffer offsets need 128 rows of C, A and B within 2 GiB.
+template <typename T, bool TRANSA, bool STORE, typename F>
+static hipError_t gemm_split(lair_context *ctx, int M, int N, int K, const T *A, int64_t lda, const T *B,
+                             int64_t ldb, T *C, int64_t ldc, F interior) {
+  const int64_t big = std::max(std::max(lda, ldb), ldc);
+  if (big > ((int64_t)1 << 31) / ((int64_t)kTileM * (int64_t)sizeof(T))) return hipErrorInvalidValue;
+  hipStream_t st = ctx->stream;
+  auto masked = [&](int m, int n, const T *a, const T *b, T *c) {
+    if (m <= 0 || n <= 0) return;
+    const unsigned g = (unsigned)(((m + kTileM - 1) / kTileM) * ((n + kTileN - 1) / kTileN));
+    gemm_tile_kernel<T, TRANSA, STORE, 1, true><<<g, 256, 0, st>>>(m, n, K, a, lda, b, ldb, c, ldc);
+  };
+  if (K % kBK != 0) {
+    masked(M, N, A, B, C);
+    return hipGetLastError();
   }
+  const int Mf = M / kTileM * kTileM, Nf = N / kTileN * kTileN;
+  if (Mf > 0 && Nf > 0) interior(Mf, Nf);
+  // Right strip (rows of the interior), then the bottom strip (all columns).
+  masked(Mf, N - Nf, A, B + Nf, C + Nf);
+  masked(M - Mf, N, TRANSA ? A + Mf : A + (int64_t)Mf * lda, B, C + (int64_t)Mf * ldc);
+  return hipGetLastError();
 }
 
 hipError_t launch_dgemm_tile(lair_context *ctx, int M, int N, int K, const double *A, int64_t lda,
                              bool transA, const double *B, int64_t ldb, double *C, int64_t ldc,
                              bool store) {
-  const int nbm = (M + kTileM - 1) / kTileM, nbn = (N + kTileN - 1) / kTileN;
   hipStream_t st = ctx->stream;
-  if (ctx->bulk_wg_cap > 0 && !transA && !store) {
-    const int ntile = nbm * ((nbn + 1) / 2);
-    const unsigned grid = (unsigned)std::min(ntile, ctx->bulk_wg_cap);
-    gemm_tile_kernel<double, false, false, 2><<<grid, 512, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
-    return hipGetLastError();
-  }
-  const unsigned grid = (unsigned)(nbm * nbn);
-  if (store) {
-    if (transA)
-      gemm_tile_kernel<double, true, true, 1><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
-    else
-      gemm_tile_kernel<double, false, true, 1><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
-  } else {
-    if (transA)
-      gemm_tile_kernel<double, true, false, 1><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
-    else
-      gemm_tile_kernel<double, false, false, 1><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
-  }
-  return hipGetLastError();
+  auto run = [&](auto ta, auto sto) {
+    constexpr bool TA = decltype(ta)::value, STO = decltype(sto)::value;
+    return gemm_split<double, TA, STO>(ctx, M, N, K, A, lda, B, ldb, C, ldc, [&](int m, int n) {
+      const int nbm = m / kTileM, nbn = n / kTileN;
+      if (ctx->bulk_wg_cap > 0 && !TA && !STO) {
+        const int ntile = nbm * ((nbn + 1) / 2);
+        const unsigned grid = (unsigned)std::min(ntile, ctx->bulk_wg_cap);
+        gemm_tile_kernel<double, false, false, 2, false><<<grid, 512, 0, st>>>(m, n, K, A, lda, B, ldb, C, ldc);
+      } else {
+        gemm_tile_kernel<double, TA, STO, 1, false><<<(unsigned)(nbm * nbn), 256, 0, st>>>(m, n, K, A, lda, B, ldb,
+                                                                                           C, ldc);
+      }
+    });
+  };
+  if (transA) return store ? run(std::true_type{}, std::true_type{}) : run(std::true_type{}, std::false_type{});
+  return store ? run(std::false_type{}, std::true_type{}) : run(std::false_type{}, std::false_type{});
 }
 
 hipError_t launch_sgemm_tile(lair_context *ctx, int M, int N, int K, const float *A, int64_t lda,
                              bool transA, const float *B, int64_t ldb, float *C, int64_t ldc) {
-  const int nbm = (M + kTileM - 1) / kTileM, nbn = (N + kTileN - 1) / kTileN;
-  if (ctx->bulk_wg_cap > 0 && !transA) {
-    const int ntile = nbm * ((nbn + 3) / 4);
-    const unsigned grid = (unsigned)std::min(ntile, ctx->bulk_wg_cap);
-    gemm_tile_kernel<float, false, false, 4><<<grid, 1024, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C,
-                                                                              ldc);
-    return hipGetLastError();
-  }
-  const unsigned grid = (unsigned)(nbm * nbn);
-  if (transA)
-    gemm_tile_kernel<float, true, false, 1><<<grid, 256, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C, ldc);
-  else
-    gemm_tile_kernel<float, false, false, 1><<<grid, 256, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C, ldc);
-  return hipGetLastError();
+  hipStream_t st = ctx->stream;
+  auto run = [&](auto ta) {
+    constexpr bool TA = decltype(ta)::value;
+    return gemm_split<float, TA, false>(ctx, M, N, K, A, lda, B, ldb, C, ldc, [&](int m, int n) {
+      const int nbm = m / kTileM, nbn = n / kTileN;
+      if (ctx->bulk_wg_cap > 0 && !TA) {
+        const int ntile = nbm * ((nbn + 3) / 4);
+        const unsigned grid = (unsigned)std::min(ntile, ctx->bulk_wg_cap);
+        gemm_tile_kernel<float, false, false, 4, false><<<grid, 1024, 0, st>>>(m, n, K, A, lda, B, ldb, C, ldc);
+      } else {
+        gemm_tile_kernel<float, TA, false, 1, false><<<(unsigned)(nbm * nbn), 256, 0, st>>>(m, n, K, A, lda, B, ldb,
+                                                                                            C, ldc);
+      }
+    });
+  };
+  return transA ? run(std::true_type{}) : run(std::false_type{});
 }
 
 // ------------------------------------------------------- BF16 splits ------
@@ -288,7 +434,7 @@ __global__ void __launch_bounds__(256 * TPW, TPW == 1 ? 2 : 1)
   constexpr int PK = BK + 8;  // padded k extent (bf16 elements)
   __shared__ __bf16 As_all[TPW][NS][kTileM][PK];
   __shared__ __bf16 Bs_all[TPW][NS][kTileN][PK];
-  const int h = threadIdx.x >> 8;
+  const int h = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8);
   auto &As = As_all[h];
   auto &Bs = Bs_all[h];
   const int t = threadIdx.x & 255, lane = t & 63, wave = t >> 6;
