@@ -890,7 +890,7 @@ lair_status lair_context_destroy(lair_context *ctx) {
   if (ctx->ev_bulk) hipEventDestroy(ctx->ev_bulk);
   for (Buffer *b : {&ctx->ws_a32, &ctx->ws_lu64, &ctx->ws_ipiv, &ctx->ws_x32, &ctx->ws_r,
                     &ctx->ws_tmp, &ctx->ws_rhs, &ctx->ws_perm, &ctx->ws_linv, &ctx->ws_uinv,
-                    &ctx->ws_flags, &ctx->ws_tmp2})
+                    &ctx->ws_flags, &ctx->ws_tmp2, &ctx->ws_cexp, &ctx->ws_split})
     if (b->ptr) hipFree(b->ptr);
   hipStreamDestroy(ctx->stream);
   if (ctx == g_default) g_default = nullptr;

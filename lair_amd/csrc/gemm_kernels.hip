@@ -335,6 +335,36 @@ __global__ void __launch_bounds__(256 * TPW, TPW == 1 ? 2 : 1)
   }
 }
 
+// Split-K products for tall-skinny C with a long K (the many-right-hand-side
+// residual): workgroup (tile, y) writes W_y = -A[:, y Ks : (y+1) Ks] B[y Ks :
+// (y+1) Ks, :] into its own slice W + y * wstride, so that every CU has work
+// although C has few tiles; the caller sums the slices.
+template <typename T>
+__global__ void __launch_bounds__(256, 2)
+    gemm_splitk_kernel(int M, int N, int K, int Ks, const T *__restrict__ A, int64_t lda, const T *__restrict__ B,
+                       int64_t ldb, T *__restrict__ W, int64_t ldw, int64_t wstride) {
+  constexpr int P = kBK + (sizeof(T) == 4 ? 4 : 2);
+  __shared__ __align__(16) T As[2][kTileM][P];
+  __shared__ __align__(16) T Bs[2][kTileN][P];
+  const int y = blockIdx.y;
+  const int k0 = y * Ks, kn = min(Ks, K - k0);
+  if (kn <= 0) return;  // uniform over the workgroup
+  const int nbn = (N + kTileN - 1) / kTileN;
+  const int tm = blockIdx.x / nbn, tn = blockIdx.x % nbn;
+  gemm_tile<T, false, true, true>(M, N, kn, A + k0, lda, B + (int64_t)k0 * ldb, ldb, W + (int64_t)y * wstride, ldw,
+                                  tm * kTileM, tn * kTileN, false, As, Bs);
+}
+
+hipError_t launch_dgemm_splitk_store(lair_context *ctx, int M, int N, int K, int S, const double *A, int64_t lda,
+                                     const double *B, int64_t ldb, double *W, int64_t ldw, int64_t wstride) {
+  if (M <= 0 || N <= 0 || K <= 0 || S <= 0) return hipSuccess;
+  const int Ks = ((K + S - 1) / S + kBK - 1) / kBK * kBK;
+  const unsigned tiles = (unsigned)(((M + kTileM - 1) / kTileM) * ((N + kTileN - 1) / kTileN));
+  gemm_splitk_kernel<double><<<dim3(tiles, (unsigned)S), 256, 0, ctx->stream>>>(M, N, K, Ks, A, lda, B, ldb, W, ldw,
+                                                                                 wstride);
+  return hipGetLastError();
+}
+
 // Splits C into the region of whole tiles (unmasked kernel, launched by
 // `interior`) and the right and bottom strips (masked kernel, one tile per
 // workgroup); K not a multiple of kBK runs masked throughout.  The 32-bit

@@ -1135,6 +1135,18 @@ __global__ void residual_kernel(int n, const double *a, int64_t lda, const doubl
   }
 }
 
+// R = B + sum over the S slices W_y (in order y = 0 .. S-1).
+__global__ void __launch_bounds__(256) splitk_sum_kernel(int n, int nrhs, int S, const double *__restrict__ W,
+                                                         int64_t wstride, const double *__restrict__ b, int64_t ldb,
+                                                         double *__restrict__ r, int64_t ldr) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (int64_t)n * nrhs) return;
+  const int i = (int)(e / nrhs), c = (int)(e % nrhs);
+  double v = b[(int64_t)i * ldb + c];
+  for (int y = 0; y < S; ++y) v += W[(int64_t)y * wstride + e];
+  r[(int64_t)i * ldr + c] = v;
+}
+
 hipError_t launch_residual(lair_context *ctx, int n, int nrhs, const double *a, int64_t lda,
                            const double *x, int64_t ldx, const double *b, int64_t ldb, double *r,
                            int64_t ldr) {
@@ -1165,10 +1177,25 @@ hipError_t launch_residual(lair_context *ctx, int n, int nrhs, const double *a, 
     prof_end(ctx, slot, LAIR_KCLASS_RESIDUAL, 2.0 * n * (double)n * nrhs);
     return hipGetLastError();
   }
-  // Many right-hand sides: R = B, then R -= A X on the FP64 MFMA path.
-  hipError_t e = launch_copy_d2d(ctx, n, nrhs, b, ldb, r, ldr);
+  // Many right-hand sides: R = B - A X on the FP64 MFMA path, split over K
+  // so that the few tiles of R still fill the chip: slices W_y = -A_y X_y,
+  // then R = B + sum_y W_y in a fixed order.
+  const int tiles = ((n + 127) / 128) * ((nrhs + 127) / 128);
+  int S = (2 * 256 + tiles - 1) / tiles;
+  S = std::max(1, std::min(S, std::min(16, (n + 255) / 256)));
+  const int64_t wstride = (int64_t)n * nrhs;
+  hipError_t e = ensure(ctx->ws_split, sizeof(double) * (size_t)wstride * S);
   if (e != hipSuccess) return e;
-  return launch_gemm_minus<double>(ctx, kGemmF64, n, nrhs, n, a, lda, false, x, ldx, r, ldr);
+  double *W = static_cast<double *>(ctx->ws_split.ptr);
+  const int slot = prof_begin(ctx, LAIR_KCLASS_RESIDUAL);
+  e = launch_dgemm_splitk_store(ctx, n, nrhs, n, S, a, lda, x, ldx, W, nrhs, wstride);
+  if (e != hipSuccess) return e;
+  const int64_t total = (int64_t)n * nrhs;
+  splitk_sum_kernel<<<(unsigned)((total + 255) / 256), 256, 0, ctx->stream>>>(n, nrhs, S, W, wstride, b, ldb, r,
+                                                                              ldr);
+  prof_end(ctx, slot, LAIR_KCLASS_RESIDUAL, 2.0 * n * (double)n * nrhs,
+           8.0 * ((double)n * n + (2.0 * S + 2.0) * n * nrhs));
+  return hipGetLastError();
 }
 
 // Column maxima of |x| as ordered bit patterns (NaN above +inf): a

@@ -159,6 +159,8 @@ struct lair_context {
   // rows of cexp_ld real values (two per column of the matrix).
   lair::Buffer ws_cexp;
   int64_t cexp_ld = 0;
+  // Split-K slices of the many-right-hand-side residual.
+  lair::Buffer ws_split;
   // When set, the triangular solves publish their tagged values here instead
   // of in ws_flags (caller-provided memory, zeroed by the caller).
   unsigned long long *xt_override = nullptr;
@@ -294,6 +296,10 @@ hipError_t launch_dgemm_tile(lair_context *ctx, int M, int N, int K, const doubl
                              bool store = false);
 hipError_t launch_sgemm_tile(lair_context *ctx, int M, int N, int K, const float *A, int64_t lda,
                              bool transA, const float *B, int64_t ldb, float *C, int64_t ldc);
+// W_y = -A[:, y Ks:(y+1) Ks] B[y Ks:(y+1) Ks, :] for y < S (Ks = K / S rounded
+// up to 16), each into W + y * wstride (leading dimension ldw).
+hipError_t launch_dgemm_splitk_store(lair_context *ctx, int M, int N, int K, int S, const double *A, int64_t lda,
+                                     const double *B, int64_t ldb, double *W, int64_t ldw, int64_t wstride);
 // ns = 3: six-product bf16 split (FP32-level); ns = 1: single bf16 product.
 hipError_t launch_bgemm_tile(lair_context *ctx, int ns, int M, int N, int K, const float *A,
                              int64_t lda, const float *B, int64_t ldb, float *C, int64_t ldc);
