@@ -181,7 +181,7 @@ def bench_adjoint(ctx, gl, steps, n=4096, nrhs=64):
             "hbm_frac": round(byts / (ms * 1e-3) / 1e9 / PEAK["hbm_gbs"], 4)}
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trailing_update_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "kernel_traffic.json")
 
 
 def measured_traffic(key):
