@@ -900,6 +900,176 @@ __global__ void __launch_bounds__(256) trsv_dataflow_kernel(TrsvArgs<T> p) {
   }
 }
 
+// Many right-hand sides in FP32 (nrhs >= 16, not transposed): the same
+// dataflow as trsv_dataflow_kernel (one workgroup per 128-row block and
+// 16-column chunk, solved values published as tagged words), with the block
+// products on matrix cores.  Wave w owns rows 32w .. 32w + 31 of the block
+// as two 16 x 16 v_mfma_f32_16x16x4f32 accumulators, so a workgroup needs
+// few registers and two fit on a CU (all 512 workgroups of a 64-column solve
+// at n = 16384 are resident at once).
+//  * E_ij tiles (128 x 128) are staged through registers into LDS, 16-byte
+//    chunk c of row r at chunk position c ^ (r & 31); MFMA step s takes
+//    k = 16 t + 4 g + s from lane group g, so a lane's four k values are one
+//    16-byte LDS read and the 16 rows read together are on distinct banks.
+//  * -X_j is staged transposed ([column][k]), so acc = b - sum E_ij X_j.
+//  * D_i^-1 (the inverted diagonal block) stays in registers in the MFMA
+//    operand layout; y goes through LDS as the second operand of x = D y.
+__global__ void __launch_bounds__(256, 2) trsm_mfma_kernel(TrsvArgs<float> p) {
+  constexpr int BS = 128, RC = 16, XP = BS + 4;
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  float *tile = reinterpret_cast<float *>(smem_raw);  // [BS][BS], swizzled 16-byte chunks
+  __shared__ __align__(16) float xs[RC][XP];          // B operand, transposed
+  __shared__ int s_bail;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int g = lane >> 4, lc = lane & 15;
+  const int ib = blockIdx.x / p.nchunks, ch = blockIdx.x % p.nchunks;
+  const int i = p.forward ? ib : p.nblk - 1 - ib;
+  const int r0 = i * BS, c0 = ch * RC;
+  const int ncols = p.nrhs - c0 < RC ? p.nrhs - c0 : RC;
+  auto swz = [](int r, int c4) { return r * BS + ((c4 ^ (r & 31)) << 2); };
+  // Rows of this lane's accumulator element (rb, r).
+  auto arow = [&](int rb, int r) { return 32 * wave + 16 * rb + 4 * g + r; };
+
+  // Staging of an E tile: thread t holds rows t/32 + 8q, chunk t % 32.
+  f4 stage[16];
+  const bool aligned = ((p.lda * (int64_t)sizeof(float)) % 16 == 0) && (reinterpret_cast<uintptr_t>(p.lu) % 16 == 0);
+  auto fetch = [&](int cb) {
+    const int cc = (t & 31) * 4;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int grow = r0 + (t >> 5) + 8 * q, gcol = cb + cc;
+      if (aligned && grow < p.n && gcol + 4 <= p.n) {
+        stage[q] = *reinterpret_cast<const f4 *>(p.lu + (int64_t)grow * p.lda + gcol);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          stage[q][e] = (grow < p.n && gcol + e < p.n) ? p.lu[(int64_t)grow * p.lda + gcol + e] : 0.0f;
+      }
+    }
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) *reinterpret_cast<f4 *>(tile + swz((t >> 5) + 8 * q, t & 31)) = stage[q];
+  };
+
+  // D_i^-1 in the MFMA A-operand layout: dreg[rb][s] = D[row(rb)][16 s + 4 g .. + 3].
+  f4 dreg[2][8];
+  {
+    const float *D = p.dinv + (size_t)i * BS * BS;
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int s = 0; s < 8; ++s)
+        dreg[rb][s] = *reinterpret_cast<const f4 *>(D + (32 * wave + 16 * rb + lc) * BS + 16 * s + 4 * g);
+  }
+  f4 acc[2];
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int gi = r0 + arow(rb, r);
+      acc[rb][r] = (gi < p.n && lc < ncols) ? p.xin[(int64_t)gi * p.ldx + c0 + lc] : 0.0f;
+    }
+  if (t == 0) s_bail = 0;
+
+  const int nb = p.forward ? i : p.nblk - 1 - i;  // dependency blocks
+  auto dep = [&](int d) { return p.forward ? d : p.nblk - 1 - d; };
+  if (nb > 0) fetch(dep(0) * BS);
+  for (int d = 0; d < nb; ++d) {
+    const int cb = dep(d) * BS;
+    __syncthreads();  // the previous product is done with tile and xs
+    stash();
+    if (d + 1 < nb) fetch(dep(d + 1) * BS);
+    // The solved values of block j: thread 0 waits on one of them first (a
+    // single spinning thread while block j is unsolved), then every thread
+    // checks its own tagged words (bounded waits).
+    if (t == 0) {
+      const unsigned long long *w0 = p.xt + ((int64_t)cb * p.nrhs + c0);
+      int spins = 0;
+      while ((uint32_t)(ld_sc1(w0) >> 32) != p.epoch) {
+        if (spins >= p.wait_limit || ((++spins & 63) == 0 && ld_sc1(p.err) != 0)) break;
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __syncthreads();
+    {
+      bool bail = false;
+#pragma unroll
+      for (int e0 = 0; e0 < BS * RC; e0 += 256) {
+        const int e = e0 + t, kr = e / RC, q = e % RC;
+        const int gi = cb + kr;
+        float v = 0.0f;
+        if (gi < p.n && q < ncols) {
+          const unsigned long long *w = p.xt + ((int64_t)gi * p.nrhs + c0 + q);
+          int spins = 0;
+          for (;;) {
+            if (spins >= p.wait_limit) {
+              bail = true;
+              break;
+            }
+            const unsigned long long x = ld_sc1(w);
+            if ((uint32_t)(x >> 32) == p.epoch) {
+              v = __uint_as_float((uint32_t)x);
+              break;
+            }
+            if ((++spins & 63) == 0 && ld_sc1(p.err) != 0) {
+              bail = true;
+              break;
+            }
+          }
+        }
+        xs[q][kr] = -v;
+      }
+      if (bail) {
+        atomic_or_agent(p.err, kErrWait);
+        s_bail = 1;
+      }
+    }
+    __syncthreads();
+    if (s_bail) return;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const f4 bq = *reinterpret_cast<const f4 *>(&xs[lc][16 * s + 4 * g]);
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) {
+        const f4 aq = *reinterpret_cast<const f4 *>(tile + swz(32 * wave + 16 * rb + lc, 4 * s + g));
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aq[ks], bq[ks], acc[rb], 0, 0, 0);
+      }
+    }
+  }
+  __syncthreads();  // the last product is done with xs
+  // y (this block's right-hand sides minus the dependencies) as the B operand.
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) xs[lc][arow(rb, r)] = acc[rb][r];
+  __syncthreads();
+  f4 out[2];
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb) out[rb] = (f4){0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    const f4 bq = *reinterpret_cast<const f4 *>(&xs[lc][16 * s + 4 * g]);
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks)
+        out[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(dreg[rb][s][ks], bq[ks], out[rb], 0, 0, 0);
+  }
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int gi = r0 + arow(rb, r);
+      if (gi < p.n && lc < ncols) {
+        p.x[(int64_t)gi * p.ldx + c0 + lc] = out[rb][r];
+        st_sc1(p.xt + ((int64_t)gi * p.nrhs + c0 + lc), tagged(p.epoch, __float_as_uint(out[rb][r])));
+      }
+    }
+}
+
 template <typename T>
 hipError_t launch_trsv_dataflow(lair_context *ctx, const T *lu, int64_t lda, const T *dinv,
                                 bool dinv_trans, const T *xin, T *x, int64_t ldx, int n, int nrhs,
@@ -908,8 +1078,10 @@ hipError_t launch_trsv_dataflow(lair_context *ctx, const T *lu, int64_t lda, con
   constexpr int BS = TrsvBlock<T>::BS;
   // Right-hand sides per workgroup: 4 columns keep each chain link short
   // (the link's products and reductions grow with the chunk width), and the
-  // chunks run as independent chains.
-  const int rc = nrhs == 1 ? 1 : 4;
+  // chunks run as independent chains.  FP32 with 16 or more right-hand sides
+  // (not transposed) runs the matrix-core kernel in chunks of 16.
+  const bool mfma = std::is_same<T, float>::value && nrhs >= 16 && !trans && !dinv_trans;
+  const int rc = mfma ? 16 : (nrhs == 1 ? 1 : 4);
   const int nblk = (n + BS - 1) / BS;
   const int nchunks = (nrhs + rc - 1) / rc;
   // Tagged words of the solved values; zeroed whenever they are (re)allocated
@@ -957,6 +1129,16 @@ hipError_t launch_trsv_dataflow(lair_context *ctx, const T *lu, int64_t lda, con
                       allow_large_lds(trsv_dataflow_kernel<T, BS, 4>), true);
   (void)once;
   const int slot = prof_begin(ctx, LAIR_KCLASS_GETRS);
+  if constexpr (std::is_same<T, float>::value) {
+    if (mfma) {
+      static bool once_m = (allow_large_lds(trsm_mfma_kernel), true);
+      (void)once_m;
+      trsm_mfma_kernel<<<grid, 256, (size_t)BS * BS * sizeof(float), ctx->stream>>>(a);
+      prof_end(ctx, slot, LAIR_KCLASS_GETRS, (double)n * n * nrhs,
+               sizeof(T) * (0.5 * (double)n * n + 2.0 * n * nrhs));
+      return hipGetLastError();
+    }
+  }
   if (rc == 1)
     trsv_dataflow_kernel<T, BS, 1><<<grid, 256, shm, ctx->stream>>>(a);
   else

@@ -337,3 +337,24 @@ def test_ties_pick_lowest_row():  # iamax.rs:15 strict '>' keeps the first maxim
     piv, _ = L.getrf(g)
     assert piv[0] == 7 == piv_o[0]
     np.testing.assert_array_equal(piv, piv_o)
+
+
+@pytest.mark.parametrize("n,nrhs", [(300, 16), (1000, 17), (2049, 64), (129, 33)])
+def test_sgetrs_many_rhs(n, nrhs):
+    """FP32 getrs with 16 or more right-hand sides (the matrix-core dataflow
+    triangular solve): backward error of order n * eps32 against the FP32
+    factors, partial 128-row blocks and partial 16-column chunks included;
+    every column equals the one-column solve to rounding."""
+    a = inputs.gaussian_matrix(5100 + n, n, n).astype(np.float32)
+    b = inputs.gaussian_matrix(5101 + n, n, nrhs).astype(np.float32)
+    g = a.copy()
+    piv, sing = L.getrf(g)
+    assert sing is None
+    x = L.getrs(g, piv, b)
+    eps32 = float(np.finfo(np.float32).eps)
+    a64, x64, b64 = a.astype(np.float64), x.astype(np.float64), b.astype(np.float64)
+    r = b64 - a64 @ x64
+    berr = (np.abs(r).max(0) / (np.abs(a64).sum(1).max() * np.abs(x64).max(0))).max()
+    assert berr <= 8 * n * eps32
+    x1 = L.getrs(g, piv, b[:, 5])
+    assert np.abs(x1 - x[:, 5]).max() <= 64 * n * eps32 * np.abs(x1).max()
