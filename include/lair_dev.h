@@ -181,6 +181,21 @@ lair_status lair_sgetrs_device(lair_context *ctx, int64_t n, int64_t nrhs, const
                                int64_t lda, const int64_t *ipiv, float *b, int64_t ldb,
                                int32_t trans);
 
+/* C -= A B on device data: the reference's gemm (src/blas/gemm.rs:6-32) with
+ * alpha = -1 and no conjugation, on the trailing-update kernels of getrf.  A
+ * is m x k (leading dimension lda >= k), B is k x n (ldb >= n), C is m x n
+ * (ldc >= n), all row-major.  The single-precision form takes prec:
+ * LAIR_PREC_FP32 (FP32 MFMA), LAIR_PREC_BF16X6 (six BF16 products, FP32
+ * accumulation) or LAIR_PREC_BF16 (one BF16 product).  The sums run in an
+ * order of the tiles' choosing, so results match the reference's loop to
+ * rounding, not bit for bit.  Returns LAIR_ERR_UNSUPPORTED for a leading
+ * dimension beyond 2^31 bytes per 128 rows.  Synchronizes the stream. */
+lair_status lair_dgemm_minus_device(lair_context *ctx, int64_t m, int64_t n, int64_t k, const double *a,
+                                    int64_t lda, const double *b, int64_t ldb, double *c, int64_t ldc);
+lair_status lair_sgemm_minus_device(lair_context *ctx, int32_t prec, int64_t m, int64_t n, int64_t k,
+                                    const float *a, int64_t lda, const float *b, int64_t ldb, float *c,
+                                    int64_t ldc);
+
 /* Complex device forms (interleaved storage, lda/ldb in complex elements,
  * pointers aligned to the complex element).  nb: panel width, 0 or < 0 the
  * default (64, the widest complex panel). */

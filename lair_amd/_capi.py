@@ -89,6 +89,8 @@ SIGNATURES = [
     ("lair_dlu_solve_backward", i32, [vp, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64]),
     ("lair_dlu_extract", i32, [vp, vp, i32, vp, i64]),
     ("lair_sgetrs_device", i32, [vp, i64, i64, vp, i64, vp, vp, i64, i32]),
+    ("lair_dgemm_minus_device", i32, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64]),
+    ("lair_sgemm_minus_device", i32, [vp, i32, i64, i64, i64, vp, i64, vp, i64, vp, i64]),
     ("lair_dgesv_mixed_device", i32,
      [vp, i64, i64, vp, i64, vp, i64, vp, i64, i32, i32, f64, ctypes.POINTER(IrInfo)]),
     ("lair_dgesv_device", i32, [vp, i64, i64, vp, i64, vp, i64, vp, i64]),

@@ -996,6 +996,55 @@ lair_status lair_sgetrs_device(lair_context *ctx, int64_t n, int64_t nrhs, const
   return st != LAIR_OK ? st : sync_status(ctx);
 }
 
+}  // extern "C"
+
+// C -= A B on device data (src/blas/gemm.rs:6-32 with alpha = -1 and no
+// conjugation), on the trailing-update kernels.
+template <typename T>
+static lair_status gemm_minus_impl(lair_context *ctx, int mode, int64_t m, int64_t n, int64_t k, const T *a,
+                                   int64_t lda, const T *b, int64_t ldb, T *c, int64_t ldc, int arg0) {
+  const int64_t lim = (int64_t)1 << 30;
+  if (m < 0 || m > lim) return -arg0;
+  if (n < 0 || n > lim) return -(arg0 + 1);
+  if (k < 0 || k > lim) return -(arg0 + 2);
+  if (m == 0 || n == 0 || k == 0) return LAIR_OK;
+  if (!a) return -(arg0 + 3);
+  if (lda < k) return -(arg0 + 4);
+  if (!b) return -(arg0 + 5);
+  if (ldb < n) return -(arg0 + 6);
+  if (!c) return -(arg0 + 7);
+  if (ldc < n) return -(arg0 + 8);
+  // The tile kernels address 128 rows of each operand with 32-bit offsets.
+  const int64_t ldmax = std::max(std::max(lda, ldb), ldc);
+  if (ldmax > ((int64_t)1 << 31) / (128 * (int64_t)sizeof(T))) {
+    set_error("gemm: leading dimension too large");
+    return LAIR_ERR_UNSUPPORTED;
+  }
+  hipError_t e = launch_gemm_minus<T>(ctx, mode, (int)m, (int)n, (int)k, a, lda, false, b, ldb, c, ldc,
+                                      LAIR_KCLASS_GEMM);
+  if (e != hipSuccess) return hip_status(e, "gemm");
+  return sync_status(ctx);
+}
+
+extern "C" {
+
+lair_status lair_dgemm_minus_device(lair_context *ctx, int64_t m, int64_t n, int64_t k, const double *a,
+                                    int64_t lda, const double *b, int64_t ldb, double *c, int64_t ldc) {
+  if (!ctx) return -1;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return gemm_minus_impl<double>(ctx, kGemmF32, m, n, k, a, lda, b, ldb, c, ldc, 2);
+}
+
+lair_status lair_sgemm_minus_device(lair_context *ctx, int32_t prec, int64_t m, int64_t n, int64_t k,
+                                    const float *a, int64_t lda, const float *b, int64_t ldb, float *c,
+                                    int64_t ldc) {
+  if (!ctx) return -1;
+  if (prec != LAIR_PREC_FP32 && prec != LAIR_PREC_BF16X6 && prec != LAIR_PREC_BF16) return -2;
+  const int mode = prec == LAIR_PREC_FP32 ? kGemmF32 : (prec == LAIR_PREC_BF16X6 ? kGemmBF16x6 : kGemmBF16);
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return gemm_minus_impl<float>(ctx, mode, m, n, k, a, lda, b, ldb, c, ldc, 3);
+}
+
 lair_status lair_zgetrf_device(lair_context *ctx, int64_t m, int64_t n, double *a, int64_t lda, int64_t *ipiv,
                                int64_t nb, int64_t *singular) {
   return cgetrf_device<zcplx>(ctx, m, n, a, lda, ipiv, nb, singular);

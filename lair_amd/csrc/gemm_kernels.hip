@@ -367,20 +367,15 @@ hipError_t launch_dgemm_splitk_store(lair_context *ctx, int M, int N, int K, int
 
 // Splits C into the region of whole tiles (unmasked kernel, launched by
 // `interior`) and the right and bottom strips (masked kernel, one tile per
-// workgroup); K not a multiple of kBK runs masked throughout.  The 32-bit
-// buffer offsets need 128 rows of C, A and B within 2 GiB.
-template <typename T, bool TRANSA, bool STORE, typename F>
-static hipError_t gemm_split(lair_context *ctx, int M, int N, int K, const T *A, int64_t lda, const T *B,
-                             int64_t ldb, T *C, int64_t ldc, F interior) {
+// workgroup, launched by `masked`); K not a multiple of the stage depth runs
+// masked throughout.  The 32-bit buffer offsets need 128 rows of C, A and B
+// within 2 GiB.
+template <typename T, bool TRANSA, typename FI, typename FM>
+static hipError_t split_tiles(int M, int N, int K, int kstep, const T *A, int64_t lda, const T *B, int64_t ldb,
+                              T *C, int64_t ldc, FI interior, FM masked) {
   const int64_t big = std::max(std::max(lda, ldb), ldc);
   if (big > ((int64_t)1 << 31) / ((int64_t)kTileM * (int64_t)sizeof(T))) return hipErrorInvalidValue;
-  hipStream_t st = ctx->stream;
-  auto masked = [&](int m, int n, const T *a, const T *b, T *c) {
-    if (m <= 0 || n <= 0) return;
-    const unsigned g = (unsigned)(((m + kTileM - 1) / kTileM) * ((n + kTileN - 1) / kTileN));
-    gemm_tile_kernel<T, TRANSA, STORE, 1, true><<<g, 256, 0, st>>>(m, n, K, a, lda, b, ldb, c, ldc);
-  };
-  if (K % kBK != 0) {
+  if (K % kstep != 0) {
     masked(M, N, A, B, C);
     return hipGetLastError();
   }
@@ -390,6 +385,18 @@ static hipError_t gemm_split(lair_context *ctx, int M, int N, int K, const T *A,
   masked(Mf, N - Nf, A, B + Nf, C + Nf);
   masked(M - Mf, N, TRANSA ? A + Mf : A + (int64_t)Mf * lda, B, C + (int64_t)Mf * ldc);
   return hipGetLastError();
+}
+
+template <typename T, bool TRANSA, bool STORE, typename F>
+static hipError_t gemm_split(lair_context *ctx, int M, int N, int K, const T *A, int64_t lda, const T *B,
+                             int64_t ldb, T *C, int64_t ldc, F interior) {
+  hipStream_t st = ctx->stream;
+  auto masked = [&](int m, int n, const T *a, const T *b, T *c) {
+    if (m <= 0 || n <= 0) return;
+    const unsigned g = (unsigned)(((m + kTileM - 1) / kTileM) * ((n + kTileN - 1) / kTileN));
+    gemm_tile_kernel<T, TRANSA, STORE, 1, true><<<g, 256, 0, st>>>(m, n, K, a, lda, b, ldb, c, ldc);
+  };
+  return split_tiles<T, TRANSA>(M, N, K, kBK, A, lda, B, ldb, C, ldc, interior, masked);
 }
 
 hipError_t launch_dgemm_tile(lair_context *ctx, int M, int N, int K, const double *A, int64_t lda,
@@ -451,183 +458,211 @@ __device__ __forceinline__ void split3(float x, __bf16 *out) {
 
 // C -= A B with FP32 operands split into NS bf16 terms.  NS = 3 keeps the six
 // products whose magnitude is at least 2^-16 of the leading one (hh, hm, mh,
-// mm, hl, lh), issued smallest first into the FP32 MFMA accumulators.
-// A is M x K row-major; B is K x N row-major.  Stage: BK = 32 (one
-// 16x16x32 MFMA step).  LDS holds the split planes [s][i][k] of A and
-// [s][j][k] of B (k contiguous: each lane reads 8 consecutive k); the next
-// stage's raw FP32 values are prefetched into registers during the MFMAs.
-template <int NS, int TPW>
+// mm, hl, lh), issued smallest first.  A is M x K row-major; B is K x N
+// row-major.
+//
+// Same plan as gemm_tile: a 128 x 128 tile per group of four waves, the
+// accumulators start from C and the MFMAs add (-A) B (A is negated as it is
+// split), buffer addressing, unmasked interior tiles and masked edge strips.
+// Each wave owns a 64 x 64 quadrant as 2 x 2 blocks of 32 x 32 accumulators
+// (v_mfma_f32_32x32x16_bf16: lane l supplies A[l%32][8(l/32) .. +8] and
+// B[8(l/32) .. +8][l%32] and holds C[8(r/4) + 4(l/32) + r%4][l%32]).  A stage
+// is BK = 16 (one MFMA k-step); LDS holds the split planes of A as [i][k] and
+// of B as [j][k], double buffered, at a pitch of 24 bf16 (48 bytes: the 16
+// rows that 16 lanes read at once fall on distinct 16-byte bank groups).  The
+// FP32 values of the next stage are fetched into registers during the MFMAs
+// and split when they are written to the other buffer.
+constexpr int kBBK = 16;
+constexpr int kBP = kBBK + 8;
+
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+template <int NS, int NSL, bool MASKED>
+__device__ __forceinline__ void bgemm_tile(int M, int N, int K, const float *__restrict__ A, int64_t lda,
+                                           const float *__restrict__ B, int64_t ldb, float *__restrict__ C,
+                                           int64_t ldc, int bm, int bn, bool ghost,
+                                           __bf16 (*As)[NSL][kTileM][kBP], __bf16 (*Bs)[NSL][kTileN][kBP]) {
+  const int t = threadIdx.x & 255, lane = t & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((threadIdx.x & 255) >> 6);
+  const int wm = (wv >> 1) * 64, wn = (wv & 1) * 64;
+
+  // C element (i, j, r) of this lane: quadrant row i*32 + 8(r/4) + r%4 past
+  // the lane's first row 4(l/32), column j*32 past the lane's column l%32.
+  const int lr0 = 4 * (lane >> 5), lc = lane & 31;
+  const int mrem = M - (bm + wm + lr0), nrem = N - (bn + wn + lc);
+  const int coff = (lr0 * (int)ldc + lc) * 4;
+  auto crow = [](int i, int r) { return i * 32 + 8 * (r >> 2) + (r & 3); };
+  const __amdgpu_buffer_rsrc_t crs = buf_rsrc(C + (int64_t)(bm + wm) * ldc + (bn + wn));
+  f16v acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        if (!MASKED) {
+          acc[i][j][r] = buf_ld<float>(crs, coff, (crow(i, r) * (int)ldc + j * 32) * 4);
+        } else {
+          const int gi = min(bm + wm + lr0 + crow(i, r), M - 1);
+          const int gj = min(bn + wn + lc + j * 32, N - 1);
+          acc[i][j][r] = C[(int64_t)gi * ldc + gj];
+        }
+      }
+
+  // Global fetch: A: thread -> row t/2, k (t%2)*8 .. +8 (two 16-byte loads);
+  // B: thread -> column t%128, k (t/128)*8 .. +8 (coalesced along the row).
+  float ra[8], rb[8];
+  auto fetch = [&](int k0) {
+    const int i = t >> 1, kk = (t & 1) * 8;
+    const int j = t & 127, kb = (t >> 7) * 8;
+    if (!MASKED) {
+      const __amdgpu_buffer_rsrc_t ars = buf_rsrc(A + (int64_t)bm * lda + k0);
+      const int avo = (i * (int)lda + kk) * 4;
+#pragma unroll
+      for (int q = 0; q < 8; q += 4) {
+        const f4 v = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(ars, avo, q * 4, 0));
+#pragma unroll
+        for (int e = 0; e < 4; ++e) ra[q + e] = v[e];
+      }
+      const __amdgpu_buffer_rsrc_t brs = buf_rsrc(B + (int64_t)k0 * ldb + bn);
+      const int bvo = (kb * (int)ldb + j) * 4;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) rb[q] = buf_ld<float>(brs, bvo, q * (int)ldb * 4);
+    } else {
+      // Clamped indices keep every load inside the matrix; rows of A and
+      // columns of B outside it only reach accumulators that are never
+      // stored, and k beyond K is zeroed in both operands.
+      const float *ar = A + (int64_t)min(bm + i, M - 1) * lda;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int gk = k0 + kk + q;
+        const float v = ar[min(gk, K - 1)];
+        ra[q] = gk < K ? v : 0.f;
+      }
+      const int cj = min(bn + j, N - 1);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int gk = k0 + kb + q;
+        const float v = B[(int64_t)min(gk, K - 1) * ldb + cj];
+        rb[q] = gk < K ? v : 0.f;
+      }
+    }
+  };
+  auto stash = [&](int buf) {
+    const int i = t >> 1, kk = (t & 1) * 8;
+    const int j = t & 127, kb = (t >> 7) * 8;
+    bf8 pa[NS], pb[NS];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      __bf16 sa[NS], sb[NS];
+      split3<NS>(-ra[e], sa);
+      split3<NS>(rb[e], sb);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        pa[s][e] = sa[s];
+        pb[s][e] = sb[s];
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      *reinterpret_cast<bf8 *>(&As[buf][s][i][kk]) = pa[s];
+      *reinterpret_cast<bf8 *>(&Bs[buf][s][j][kb]) = pb[s];
+    }
+  };
+
+  const int nstage = (K + kBBK - 1) / kBBK;
+  const int lrow = lane & 31, lk = (lane >> 5) * 8;
+  fetch(0);
+  stash(0);
+  __syncthreads();
+  for (int s = 0; s < nstage; ++s) {
+    const int buf = s & 1;
+    if (s + 1 < nstage) fetch((s + 1) * kBBK);
+    bf8 av[NS][2], bv[NS][2];
+#pragma unroll
+    for (int p = 0; p < NS; ++p)
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        av[p][i] = *reinterpret_cast<const bf8 *>(&As[buf][p][wm + i * 32 + lrow][lk]);
+        bv[p][i] = *reinterpret_cast<const bf8 *>(&Bs[buf][p][wn + i * 32 + lrow][lk]);
+      }
+    // Products smallest first, each over the four independent blocks.
+#pragma unroll
+    for (int p = 0; p < (NS == 3 ? 6 : 1); ++p) {
+      const int qa = NS == 3 ? (p == 0 ? 2 : p == 1 ? 0 : p == 2 ? 1 : p == 3 ? 1 : 0) : 0;
+      const int qb = NS == 3 ? (p == 0 ? 0 : p == 1 ? 2 : p == 2 ? 1 : p == 3 ? 0 : p == 4 ? 1 : 0) : 0;
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[qa][i], bv[qb][j], acc[i][j], 0, 0, 0);
+    }
+    if (s + 1 < nstage) stash(buf ^ 1);
+    __syncthreads();
+  }
+
+  int ldc_e = (int)ldc, mrem_e = mrem, nrem_e = nrem;
+  asm volatile("" : "+s"(ldc_e));
+  asm volatile("" : "+v"(mrem_e), "+v"(nrem_e));
+  if (ghost) return;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        if (!MASKED || (crow(i, r) < mrem_e && j * 32 < nrem_e))
+          buf_st<float>(acc[i][j][r], crs, coff, (crow(i, r) * ldc_e + j * 32) * 4);
+      }
+}
+
+// TPW = 2 (capped grids): two horizontally adjacent tiles per workgroup, as
+// gemm_tile_kernel.  NSL >= 2 planes are allocated so that the workgroup's
+// LDS (96 KB or more) keeps it alone on its CU.
+template <int NS, int TPW, bool MASKED>
 __global__ void __launch_bounds__(256 * TPW, TPW == 1 ? 2 : 1)
     bgemm_tile_kernel(int M, int N, int K, const float *__restrict__ A, int64_t lda,
                       const float *__restrict__ B, int64_t ldb, float *__restrict__ C, int64_t ldc) {
-  constexpr int BK = 32;
-  constexpr int PK = BK + 8;  // padded k extent (bf16 elements)
-  __shared__ __bf16 As_all[TPW][NS][kTileM][PK];
-  __shared__ __bf16 Bs_all[TPW][NS][kTileN][PK];
+  constexpr int NSL = (TPW > 1 && NS < 2) ? 2 : NS;
+  __shared__ __align__(16) __bf16 As_all[TPW][2][NSL][kTileM][kBP];
+  __shared__ __align__(16) __bf16 Bs_all[TPW][2][NSL][kTileN][kBP];
   const int h = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8);
-  auto &As = As_all[h];
-  auto &Bs = Bs_all[h];
-  const int t = threadIdx.x & 255, lane = t & 63, wave = t >> 6;
   const int nbm = (M + kTileM - 1) / kTileM, nbn = (N + kTileN - 1) / kTileN;
   const int nbnp = (nbn + TPW - 1) / TPW;
   const int ntile = nbm * nbnp;
-  // A capped grid loops over the tiles (same XCD grouping: gridDim.x is a
-  // multiple of 8 whenever it is below the tile count).
   for (int id = blockIdx.x; id < ntile; id += gridDim.x) {
-  int tm, tnp;
-  tile_coords(id, nbm, nbnp, tm, tnp);
-  const int tn = tnp * TPW + h;
-  const int bm = tm * kTileM, bn = tn * kTileN;
-  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
-  // A: thread -> row t/2, k (t%2)*16 .. +16.  B: thread -> j 4*(t%32) .. +4,
-  // k rows 2*(t/32) + {0, 1} and + 16.
-  float ra[16], rb[16];
-  auto fetch = [&](int k0) {
-    const int i = t >> 1, kk = (t & 1) * 16;
-    const int gi = bm + i;
-#pragma unroll
-    for (int q = 0; q < 16; q += 4) {
-      const int gk = k0 + kk + q;
-      if (gi < M && gk + 3 < K) {
-        const f4 v = *reinterpret_cast<const f4 *>(A + (int64_t)gi * lda + gk);
-        ra[q] = v[0];
-        ra[q + 1] = v[1];
-        ra[q + 2] = v[2];
-        ra[q + 3] = v[3];
-      } else {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) ra[q + e] = (gi < M && gk + e < K) ? A[(int64_t)gi * lda + gk + e] : 0.f;
-      }
-    }
-    const int j0 = (t & 31) * 4, kp = (t >> 5) * 2;
-#pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-      for (int e = 0; e < 2; ++e) {
-        const int gk = k0 + kp + e + 16 * h;
-        const int gj = bn + j0;
-        float *dst = rb + (h * 2 + e) * 4;
-        if (gk < K && gj + 3 < N) {
-          const f4 v = *reinterpret_cast<const f4 *>(B + (int64_t)gk * ldb + gj);
-          dst[0] = v[0];
-          dst[1] = v[1];
-          dst[2] = v[2];
-          dst[3] = v[3];
-        } else {
-#pragma unroll
-          for (int q = 0; q < 4; ++q) dst[q] = (gk < K && gj + q < N) ? B[(int64_t)gk * ldb + gj + q] : 0.f;
-        }
-      }
-  };
-  auto stash = [&]() {
-    const int i = t >> 1, kk = (t & 1) * 16;
-#pragma unroll
-    for (int q = 0; q < 16; q += 8) {
-      bf8 part[NS];
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        __bf16 sp[NS];
-        split3<NS>(ra[q + e], sp);
-#pragma unroll
-        for (int s = 0; s < NS; ++s) part[s][e] = sp[s];
-      }
-#pragma unroll
-      for (int s = 0; s < NS; ++s) *reinterpret_cast<bf8 *>(&As[s][i][kk + q]) = part[s];
-    }
-    const int j0 = (t & 31) * 4, kp = (t >> 5) * 2;
-#pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        __bf16 s0[NS], s1[NS];
-        split3<NS>(rb[(h * 2 + 0) * 4 + q], s0);
-        split3<NS>(rb[(h * 2 + 1) * 4 + q], s1);
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-          typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
-          *reinterpret_cast<bf2 *>(&Bs[s][j0 + q][kp + 16 * h]) = (bf2){s0[s], s1[s]};
-        }
-      }
-  };
-
-  f4 acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
-
-  const int nstage = (K + BK - 1) / BK;
-  fetch(0);
-  for (int s = 0; s < nstage; ++s) {
-    __syncthreads();
-    stash();
-    __syncthreads();
-    if (s + 1 < nstage) fetch((s + 1) * BK);
-    bf8 av[NS][4], bv[NS][4];
-#pragma unroll
-    for (int p = 0; p < NS; ++p) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-        av[p][i] = *reinterpret_cast<const bf8 *>(&As[p][wm + i * 16 + (lane & 15)][(lane >> 4) * 8]);
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        bv[p][j] = *reinterpret_cast<const bf8 *>(&Bs[p][wn + j * 16 + (lane & 15)][(lane >> 4) * 8]);
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        if (NS == 3) {
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[2][i], bv[0][j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[0][i], bv[2][j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[1][i], bv[1][j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[1][i], bv[0][j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[0][i], bv[1][j], acc[i][j], 0, 0, 0);
-        }
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[0][i], bv[0][j], acc[i][j], 0, 0, 0);
-      }
-  }
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    float cv[4][4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int gi = bm + wm + i * 16 + 4 * (lane >> 4) + r;
-        const int gj = bn + wn + j * 16 + (lane & 15);
-        cv[j][r] = (gi < M && gj < N) ? C[(int64_t)gi * ldc + gj] : 0.f;
-      }
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int gi = bm + wm + i * 16 + 4 * (lane >> 4) + r;
-        const int gj = bn + wn + j * 16 + (lane & 15);
-        if (gi < M && gj < N) C[(int64_t)gi * ldc + gj] = cv[j][r] - acc[i][j][r];
-      }
-  }
+    int tm, tnp;
+    tile_coords(id, nbm, nbnp, tm, tnp);
+    const int tn = tnp * TPW + h;
+    const bool ghost = tn >= nbn;
+    const int bm = tm * kTileM, bn = (ghost && !MASKED ? nbn - 1 : tn) * kTileN;
+    bgemm_tile<NS, NSL, MASKED>(M, N, K, A, lda, B, ldb, C, ldc, bm, bn, ghost, As_all[h], Bs_all[h]);
   }
 }
 
 hipError_t launch_bgemm_tile(lair_context *ctx, int ns, int M, int N, int K, const float *A,
                              int64_t lda, const float *B, int64_t ldb, float *C, int64_t ldc) {
-  const int nbm = (M + kTileM - 1) / kTileM, nbn = (N + kTileN - 1) / kTileN;
-  if (ctx->bulk_wg_cap > 0) {
-    const int ntile = nbm * ((nbn + 1) / 2);
-    const unsigned grid = (unsigned)std::min(ntile, ctx->bulk_wg_cap);
-    if (ns == 3)
-      bgemm_tile_kernel<3, 2><<<grid, 512, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C, ldc);
-    else
-      bgemm_tile_kernel<1, 2><<<grid, 512, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C, ldc);
-    return hipGetLastError();
-  }
-  const unsigned grid = (unsigned)(nbm * nbn);
-  if (ns == 3)
-    bgemm_tile_kernel<3, 1><<<grid, 256, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C, ldc);
-  else
-    bgemm_tile_kernel<1, 1><<<grid, 256, 0, ctx->stream>>>(M, N, K, A, lda, B, ldb, C, ldc);
-  return hipGetLastError();
+  hipStream_t st = ctx->stream;
+  auto run = [&](auto nsc) {
+    constexpr int NS = decltype(nsc)::value;
+    auto masked = [&](int m, int n, const float *a, const float *b, float *c) {
+      if (m <= 0 || n <= 0) return;
+      const unsigned g = (unsigned)(((m + kTileM - 1) / kTileM) * ((n + kTileN - 1) / kTileN));
+      bgemm_tile_kernel<NS, 1, true><<<g, 256, 0, st>>>(m, n, K, a, lda, b, ldb, c, ldc);
+    };
+    auto interior = [&](int m, int n) {
+      const int nbm = m / kTileM, nbn = n / kTileN;
+      if (ctx->bulk_wg_cap > 0) {
+        const int ntile = nbm * ((nbn + 1) / 2);
+        const unsigned grid = (unsigned)std::min(ntile, ctx->bulk_wg_cap);
+        bgemm_tile_kernel<NS, 2, false><<<grid, 512, 0, st>>>(m, n, K, A, lda, B, ldb, C, ldc);
+      } else {
+        bgemm_tile_kernel<NS, 1, false><<<(unsigned)(nbm * nbn), 256, 0, st>>>(m, n, K, A, lda, B, ldb, C, ldc);
+      }
+    };
+    return split_tiles<float, false>(M, N, K, kBBK, A, lda, B, ldb, C, ldc, interior, masked);
+  };
+  return ns == 3 ? run(std::integral_constant<int, 3>{}) : run(std::integral_constant<int, 1>{});
 }
 
 }  // namespace lair
