@@ -206,6 +206,10 @@ __global__ void __launch_bounds__(64 * (NWK + 1)) rpanel_kernel(RpArgs<T> pa) {
   int cpos[RPT];
   bool act[RPT];
   T lr[RPT];
+  // Entry j of each row (all updates through pivot j - 1) and entry j + 1
+  // after the pending update, kept in registers between the phases that
+  // produce and use them.
+  T cj[RPT], c1[RPT];
 #pragma unroll
   for (int s = 0; s < RPT; ++s) {
     cpos[s] = base + tw + s * NW;
@@ -215,8 +219,27 @@ __global__ void __launch_bounds__(64 * (NWK + 1)) rpanel_kernel(RpArgs<T> pa) {
   for (int e = t; e < NB; e += NT) s_at[e] = pa.row0 + e;
   if (t == 0) s_bail = 0;
   __syncthreads();
+#pragma unroll
+  for (int s = 0; s < RPT; ++s) {
+    cj[s] = wave > 0 ? s_a[tw + s * NW] : T(0);
+    c1[s] = T(0);
+  }
 
   bool pend = false;  // the previous pivot's update of columns > j is pending
+  // Wave 0: the winning row's entries j + 1 .. NB - 1 and its multiplier
+  // entry j - 1 (tagged words; columns past the row read as valid zeros).
+  unsigned long long rw[NI][WPE], mw[WPE];
+  auto load_row = [&](int q, int jj, uint32_t tag, bool pnd) {
+    const unsigned long long *rs = pa.rowd + ((size_t)(jj & 1) * kMaxPanelWG + q) * kRpRowWords;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const int c = jj + 1 + lane + 64 * i;
+#pragma unroll
+      for (int p = 0; p < WPE; ++p) rw[i][p] = c < NB ? ld_agent(rs + c * WPE + p) : tagged(tag, 0);
+    }
+#pragma unroll
+    for (int p = 0; p < WPE; ++p) mw[p] = pnd ? ld_agent(rs + (jj - 1) * WPE + p) : tagged(tag, 0);
+  };
   int j = 0;
   for (; j < pa.npiv; ++j) {
     const int par = j & 1;
@@ -232,7 +255,7 @@ __global__ void __launch_bounds__(64 * (NWK + 1)) rpanel_kernel(RpArgs<T> pa) {
 #pragma unroll
       for (int s = 0; s < RPT; ++s) {
         if (act[s]) {
-          const double x = rp_key(s_a[j * LDC + tw + s * NW]);
+          const double x = rp_key(cj[s]);
           if (x > bv || (x == bv && x > 0.0 && cpos[s] < bp)) {
             bv = x;
             bp = cpos[s];
@@ -327,6 +350,13 @@ __global__ void __launch_bounds__(64 * (NWK + 1)) rpanel_kernel(RpArgs<T> pa) {
       int gp;
       wave_argmax(hv, hp, gv, gp);
       const bool none = gp == INT_MAX;
+      // C4. The winner's row loads are issued now and stay in flight
+      //     through the bookkeeping and the barrier (C5 completes them).
+      int q = 0;
+      if (!none && !bail) {
+        q = __ffsll((long long)__ballot(lane < pa.P && hp == gp)) - 1;
+        load_row(q, j, tg, pend);
+      }
       if (lane == 0) {
         const int occ = s_at[j];
         const int pw = none ? d : gp;
@@ -344,7 +374,6 @@ __global__ void __launch_bounds__(64 * (NWK + 1)) rpanel_kernel(RpArgs<T> pa) {
         }
       }
       if (!none) {
-        const int q = __ffsll((long long)__ballot(lane < pa.P && hp == gp)) - 1;
         const T v = readlane_t(hvs, q), u1 = readlane_t(hu1, q);
         if (lane == 0) {
           s_recip[par] = rp_recip(v);
@@ -363,6 +392,12 @@ __global__ void __launch_bounds__(64 * (NWK + 1)) rpanel_kernel(RpArgs<T> pa) {
           T *row = s_a + tw + s * NW;
           const T l = lr[s];
           int c = j + 1;
+          if (c < ncols) {
+            const T e = l * u[c];
+            c1[s] = row[c * LDC] - e;
+            row[c * LDC] = c1[s];
+            ++c;
+          }
           for (; c + 8 <= ncols; c += 8) {
             T x[8], y[8];
 #pragma unroll
@@ -392,47 +427,37 @@ __global__ void __launch_bounds__(64 * (NWK + 1)) rpanel_kernel(RpArgs<T> pa) {
       //     pending update (same operands, same order), for the next
       //     iteration's update; read while the workers go on.
       if (!none) {
-        const unsigned long long *rs = pa.rowd + ((size_t)par * kMaxPanelWG + s_q[par]) * kRpRowWords;
-        T rv[NI];
-        T mult = T(0);
         bool bail = false;
         int spins = 0;
         for (;;) {
-          if (spins >= pa.wait_limit) {
-            bail = true;
-            break;
-          }
           bool ok = true;
 #pragma unroll
-          for (int i = 0; i < NI; ++i) {
-            const int c = j + 1 + lane + 64 * i;
-            rv[i] = T(0);
-            if (c < NB) {
-              uint32_t w[WPE];
+          for (int i = 0; i < NI; ++i)
 #pragma unroll
-              for (int p = 0; p < WPE; ++p) {
-                const unsigned long long x = ld_agent(rs + c * WPE + p);
-                ok = ok && (uint32_t)(x >> 32) == tg;
-                w[p] = (uint32_t)x;
-              }
-              rv[i] = from_words<T>(w);
-            }
-          }
-          if (pend) {
-            uint32_t w[WPE];
+            for (int p = 0; p < WPE; ++p) ok = ok && (uint32_t)(rw[i][p] >> 32) == tg;
 #pragma unroll
-            for (int p = 0; p < WPE; ++p) {
-              const unsigned long long x = ld_agent(rs + (j - 1) * WPE + p);
-              ok = ok && (uint32_t)(x >> 32) == tg;
-              w[p] = (uint32_t)x;
-            }
-            mult = from_words<T>(w);
-          }
+          for (int p = 0; p < WPE; ++p) ok = ok && (uint32_t)(mw[p] >> 32) == tg;
           if (__all(ok)) break;
-          if ((++spins & 63) == 0 && ld_agent(pa.err) != 0) {
+          if (spins >= pa.wait_limit || ((++spins & 63) == 0 && ld_agent(pa.err) != 0)) {
             bail = true;
             break;
           }
+          load_row(s_q[par], j, tg, pend);
+        }
+        T rv[NI];
+        T mult = T(0);
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+          uint32_t w[WPE];
+#pragma unroll
+          for (int p = 0; p < WPE; ++p) w[p] = (uint32_t)rw[i][p];
+          rv[i] = from_words<T>(w);
+        }
+        if (pend) {
+          uint32_t w[WPE];
+#pragma unroll
+          for (int p = 0; p < WPE; ++p) w[p] = (uint32_t)mw[p];
+          mult = from_words<T>(w);
         }
         if (bail) {
           // The status word is set, so the next exchange gives up at once
@@ -471,19 +496,23 @@ __global__ void __launch_bounds__(64 * (NWK + 1)) rpanel_kernel(RpArgs<T> pa) {
           cpos[s] = pw;
         }
       }
-      if (!none) {
-        const T recip = s_recip[par], u1 = s_u1[par];
+      const T recip = s_recip[par], u1 = s_u1[par];
 #pragma unroll
-        for (int s = 0; s < RPT; ++s) {
-          if (act[s]) {
-            T *row = s_a + tw + s * NW;
-            const T l = row[j * LDC] * recip;
+      for (int s = 0; s < RPT; ++s) {
+        if (act[s]) {
+          T *row = s_a + tw + s * NW;
+          const T v1 = (j + 1 < ncols) ? (pend ? c1[s] : row[(j + 1) * LDC]) : T(0);
+          if (!none) {
+            const T l = cj[s] * recip;
             row[j * LDC] = l;
             lr[s] = l;
             if (j + 1 < ncols) {
               const T e = l * u1;
-              row[(j + 1) * LDC] = row[(j + 1) * LDC] - e;
+              cj[s] = v1 - e;
+              row[(j + 1) * LDC] = cj[s];
             }
+          } else {
+            cj[s] = v1;
           }
         }
       }
