@@ -797,18 +797,9 @@ __global__ void __launch_bounds__(256) trsv_dataflow_kernel(TrsvArgs<T> p) {
     const int cb = j * BS;
     stash(stage, p.trans != 0);
     if (d + 1 < nb) issue(d + 1, stage);
-    // The solved values of block j.  Thread 0 first waits on one of them, so
-    // that a single thread per workgroup spins while block j is unsolved;
-    // then every thread checks its own tagged words (bounded waits).
-    if (t == 0) {
-      const unsigned long long *w0 = p.xt + ((int64_t)cb * p.nrhs + c0) * WPE;
-      int spins = 0;
-      while ((uint32_t)(ld_sc1(w0) >> 32) != p.epoch) {
-        if (spins >= p.wait_limit || ((++spins & 63) == 0 && ld_sc1(p.err) != 0)) break;
-        __builtin_amdgcn_s_sleep(1);
-      }
-    }
-    __syncthreads();
+    // The solved values of block j: every thread polls its own tagged words
+    // (bounded waits), so the values arrive with the first load that sees
+    // them.
     {
       bool bail = false;
       for (int e = t; e < BS * RC; e += 256) {
@@ -839,6 +830,7 @@ __global__ void __launch_bounds__(256) trsv_dataflow_kernel(TrsvArgs<T> p) {
               bail = true;
               break;
             }
+            __builtin_amdgcn_s_sleep(1);
           }
         }
         xs[rr][q] = v;
@@ -981,18 +973,8 @@ __global__ void __launch_bounds__(256, 2) trsm_mfma_kernel(TrsvArgs<float> p) {
     __syncthreads();  // the previous product is done with tile and xs
     stash();
     if (d + 1 < nb) fetch(dep(d + 1) * BS);
-    // The solved values of block j: thread 0 waits on one of them first (a
-    // single spinning thread while block j is unsolved), then every thread
-    // checks its own tagged words (bounded waits).
-    if (t == 0) {
-      const unsigned long long *w0 = p.xt + ((int64_t)cb * p.nrhs + c0);
-      int spins = 0;
-      while ((uint32_t)(ld_sc1(w0) >> 32) != p.epoch) {
-        if (spins >= p.wait_limit || ((++spins & 63) == 0 && ld_sc1(p.err) != 0)) break;
-        __builtin_amdgcn_s_sleep(1);
-      }
-    }
-    __syncthreads();
+    // The solved values of block j: every thread polls its own tagged words
+    // (bounded waits).
     {
       bool bail = false;
 #pragma unroll
@@ -1017,6 +999,7 @@ __global__ void __launch_bounds__(256, 2) trsm_mfma_kernel(TrsvArgs<float> p) {
               bail = true;
               break;
             }
+            __builtin_amdgcn_s_sleep(1);
           }
         }
         xs[q][kr] = -v;
