@@ -47,58 +47,50 @@ static void allow_large_lds(F *kernel) {
 
 // ------------------------------------------------------------- rowmap ---
 
-// rows[e] <- contents of src[e] (before the interchanges), e < *count.
-__global__ void __launch_bounds__(64)
+// rows[e] <- contents of src[e] (before the interchanges), e < *count, for
+// the interchanges ipiv[k0 .. k1) applied in order (k swaps with ipiv[k]).
+// The rows involved are the block's rows k0 .. k1 - 1 and the distinct pivot
+// rows outside it.  Each thread takes one such row r and finds its source by
+// tracing r back through the interchanges, last to first (a swap of k and p
+// maps k to p and p to k), so the rows are resolved independently of each
+// other.
+__global__ void __launch_bounds__(256)
     rowmap_kernel(const int64_t *ipiv, int k0, int k1, int *rows, int *src, int *count) {
-  __shared__ int s_rows[kMaxSwapRows], s_src[kMaxSwapRows], s_piv[kMaxSwapRows / 2];
-  __shared__ int s_nx;
-  const int lane = threadIdx.x;
+  __shared__ int s_piv[kMaxSwapRows / 2], s_out[kMaxSwapRows / 2];
+  __shared__ int s_wcnt[4];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int kb = k1 - k0;
-  for (int e = lane; e < kb; e += 64) {
-    s_rows[e] = k0 + e;
-    s_src[e] = k0 + e;
-    s_piv[e] = (int)ipiv[k0 + e];
-  }
-  if (lane == 0) s_nx = 0;
+  for (int e = t; e < kb; e += 256) s_piv[e] = (int)ipiv[k0 + e];
   __syncthreads();
-  for (int e = 0; e < kb; ++e) {
-    const int k = k0 + e;
-    const int p = s_piv[e];
-    if (p == k) continue;
-    int ep;
-    if (p >= k0 && p < k1) {
-      ep = p - k0;
-    } else {
-      const int nx = s_nx;
-      ep = -1;
-      for (int b = 0; b < nx && ep < 0; b += 64) {
-        const bool hit = (b + lane < nx) && s_rows[kb + b + lane] == p;
-        const unsigned long long m = __ballot(hit);
-        if (m) ep = kb + b + __ffsll((long long)m) - 1;
-      }
-      if (ep < 0) {
-        ep = kb + nx;
-        if (lane == 0) {
-          s_rows[ep] = p;
-          s_src[ep] = p;
-          s_nx = nx + 1;
-        }
-      }
-      __syncthreads();
-    }
-    if (lane == 0) {
-      const int tmp = s_src[e];
-      s_src[e] = s_src[ep];
-      s_src[ep] = tmp;
-    }
-    __syncthreads();
+  // Pivot rows outside the block, each listed once (first occurrence).
+  int p = 0;
+  bool first = false;
+  if (t < kb) {
+    p = s_piv[t];
+    first = p < k0 || p >= k1;
+    for (int e = 0; e < t && first; ++e) first = s_piv[e] != p;
   }
-  const int total = kb + s_nx;
-  for (int e = lane; e < total; e += 64) {
-    rows[e] = s_rows[e];
-    src[e] = s_src[e];
+  const unsigned long long m = __ballot(first);
+  if (lane == 0) s_wcnt[wv] = __popcll(m);
+  __syncthreads();
+  int base = 0;
+  for (int w = 0; w < wv; ++w) base += s_wcnt[w];
+  if (first) s_out[base + __popcll(m & ((1ull << lane) - 1))] = p;
+  const int nout = s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
+  __syncthreads();
+  const int total = kb + nout;
+  for (int idx = t; idx < total; idx += 256) {
+    const int r = idx < kb ? k0 + idx : s_out[idx - kb];
+    int x = r;
+#pragma unroll 8
+    for (int e = kb - 1; e >= 0; --e) {
+      const int pe = s_piv[e], ke = k0 + e;
+      x = x == ke ? pe : (x == pe ? ke : x);
+    }
+    rows[idx] = r;
+    src[idx] = x;
   }
-  if (lane == 0) *count = total;
+  if (t == 0) *count = total;
 }
 
 // Columns [c0, c0+n1) and [s1, s1 + total-n1): 32 per workgroup.
@@ -177,7 +169,7 @@ hipError_t launch_rowswap(lair_context *ctx, T *a, int64_t lda, int c0, int c1, 
   if (k1 <= k0) return hipSuccess;
   if (k1 - k0 > kMaxSwapRows / 2) return hipErrorInvalidValue;
   const int slot = prof_begin(ctx, LAIR_KCLASS_LASWP);
-  rowmap_kernel<<<1, 64, 0, ctx->stream>>>(ipiv, k0, k1, ctx->swap_rows, ctx->swap_src,
+  rowmap_kernel<<<1, 256, 0, ctx->stream>>>(ipiv, k0, k1, ctx->swap_rows, ctx->swap_src,
                                            ctx->swap_count);
   hipError_t e = rowgather<T>(ctx, a, lda, c0, c1, s0, s1, ctx->swap_rows, ctx->swap_src, ctx->swap_count,
                               2 * (k1 - k0), perm);
@@ -563,7 +555,7 @@ template hipError_t launch_swap_trsm<ccplx>(lair_context *, ccplx *, int64_t, in
 hipError_t launch_rowmap(lair_context *ctx, const int64_t *ipiv, int k0, int k1) {
   if (k1 <= k0) return hipSuccess;
   if (k1 - k0 > kMaxSwapRows / 2) return hipErrorInvalidValue;
-  rowmap_kernel<<<1, 64, 0, ctx->stream>>>(ipiv, k0, k1, ctx->swap_rows, ctx->swap_src, ctx->swap_count);
+  rowmap_kernel<<<1, 256, 0, ctx->stream>>>(ipiv, k0, k1, ctx->swap_rows, ctx->swap_src, ctx->swap_count);
   return hipGetLastError();
 }
 
