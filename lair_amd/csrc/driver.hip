@@ -521,6 +521,9 @@ static double bits_to_double(unsigned long long b) {
 
 // max over columns of ||b - A x||inf / (||A||inf ||x||inf) with r = b - Ax
 // already in r.  norms must hold 1 + 2*nrhs words; norms[0] = ||A||inf bits.
+static lair_status backward_error_read(lair_context *ctx, int nrhs, const unsigned long long *norms,
+                                       double *berr);
+
 static lair_status backward_error_from(lair_context *ctx, int n, int nrhs, const double *r,
                                        int64_t ldr, const double *x, int64_t ldx,
                                        unsigned long long *norms, double *berr) {
@@ -530,6 +533,13 @@ static lair_status backward_error_from(lair_context *ctx, int n, int nrhs, const
   if (e != hipSuccess) return hip_status(e, "colmax");
   e = launch_colmax(ctx, n, nrhs, x, ldx, norms + 1 + nrhs);
   if (e != hipSuccess) return hip_status(e, "colmax");
+  return backward_error_read(ctx, nrhs, norms, berr);
+}
+
+// The worst column's ||r||/(||A|| ||x||) from norms = [||A||, ||r_c||..., ||x_c||...]
+// (bit patterns on the device); synchronizes the stream.
+static lair_status backward_error_read(lair_context *ctx, int nrhs, const unsigned long long *norms,
+                                       double *berr) {
   std::vector<unsigned long long> h(1 + 2 * (size_t)nrhs);
   LAIR_HIP(hipMemcpyAsync(h.data(), norms, h.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
   lair_status st = sync_status(ctx);
@@ -1363,10 +1373,14 @@ lair_status lair_dgesv_mixed_device(lair_context *ctx, int64_t n64, int64_t nrhs
   bool converged = false;
   int it = 0;
   double be = INFINITY;
+  // The residual kernel folds in the norms of r and x; their words are
+  // cleared here once and then by each correction's x += d.
+  e = launch_fill_f64(ctx, reinterpret_cast<double *>(norms + 1), 2 * nrhs, 0.0);
+  if (e != hipSuccess) return hip_status(e, "fill");
   for (;;) {
-    e = launch_residual(ctx, n, nrhs, a, lda, x, ldx, b, ldb, r, nrhs);
+    e = launch_residual(ctx, n, nrhs, a, lda, x, ldx, b, ldb, r, nrhs, norms + 1, norms + 1 + nrhs);
     if (e != hipSuccess) return hip_status(e, "residual");
-    st = backward_error_from(ctx, n, nrhs, r, nrhs, x, ldx, norms, &be);
+    st = backward_error_read(ctx, nrhs, norms, &be);
     if (st != LAIR_OK) return st;
     if (it < 32) info->history[it] = be;
     if (be <= tol) {
@@ -1377,7 +1391,7 @@ lair_status lair_dgesv_mixed_device(lair_context *ctx, int64_t n64, int64_t nrhs
     // Correction d = LU^-1 r in the factorization precision, x += d in FP64.
     st = lowprec_solve(r, nrhs);
     if (st != LAIR_OK) return st;
-    e = launch_axpy_s2d(ctx, n, nrhs, d32, nrhs, x, ldx);
+    e = launch_axpy_s2d(ctx, n, nrhs, d32, nrhs, x, ldx, norms + 1, 2 * nrhs);
     if (e != hipSuccess) return hip_status(e, "axpy");
     ++it;
   }

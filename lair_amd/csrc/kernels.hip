@@ -1090,48 +1090,102 @@ hipError_t launch_transpose_d(lair_context *ctx, int rows, int cols, const doubl
 }
 
 __global__ void axpy_s2d_kernel(int rows, int cols, const float *d, int64_t ldd, double *x,
-                                int64_t ldx) {
+                                int64_t ldx, unsigned long long *zero, int nz) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t total = (int64_t)rows * cols;
+  if (zero && i < nz) zero[i] = 0ull;
   if (i >= total) return;
   const int r = (int)(i / cols), c = (int)(i % cols);
   x[(int64_t)r * ldx + c] += (double)d[(int64_t)r * ldd + c];
 }
 
 hipError_t launch_axpy_s2d(lair_context *ctx, int rows, int cols, const float *d, int64_t ldd,
-                           double *x, int64_t ldx) {
-  const int64_t total = (int64_t)rows * cols;
+                           double *x, int64_t ldx, unsigned long long *zero, int nz) {
+  const int64_t total = std::max<int64_t>((int64_t)rows * cols, nz);
   if (total <= 0) return hipSuccess;
   axpy_s2d_kernel<<<(unsigned)((total + 255) / 256), 256, 0, ctx->stream>>>(rows, cols, d, ldd, x,
-                                                                            ldx);
+                                                                            ldx, zero, nz);
   return hipGetLastError();
 }
 
 // R = B - A X for up to 4 right-hand sides in one pass over A: one wave per
-// row, FP64 accumulation.
+// row, FP64 accumulation.  With rbits / xbits, the column maxima of |R| and
+// |X| are folded in as ordered bit patterns (NaN above +inf; the words start
+// at zero): each row's |r| by one atomic, |X| by the first wave, which reads
+// all of X anyway.
 template <int NR>
-__global__ void residual_kernel(int n, const double *a, int64_t lda, const double *x, int64_t ldx,
-                                const double *b, int64_t ldb, double *r, int64_t ldr) {
-  const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (row >= n) return;
-  const double *arow = a + (int64_t)row * lda;
+__global__ void __launch_bounds__(256)
+    residual_kernel(int n, const double *a, int64_t lda, const double *x, int64_t ldx, const double *b,
+                    int64_t ldb, double *r, int64_t ldr, unsigned long long *rbits, unsigned long long *xbits) {
+  __shared__ unsigned long long s_rm[4][NR];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row = blockIdx.x * 4 + wave;
+  const bool has = row < n;
+  const bool xmax = xbits && row == 0;
   double s[NR];
-#pragma unroll
-  for (int q = 0; q < NR; ++q) s[q] = 0.0;
-  for (int c = lane; c < n; c += 64) {
-    const double av = arow[c];
-#pragma unroll
-    for (int q = 0; q < NR; ++q) s[q] += av * x[(int64_t)c * ldx + q];
-  }
+  unsigned long long xm[NR];
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
+    s[q] = 0.0;
+    xm[q] = 0ull;
+  }
+  if (has) {
+    const double *arow = a + (int64_t)row * lda;
+    if (xmax) {
+      for (int c = lane; c < n; c += 64) {
+        const double av = arow[c];
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) s[q] += __shfl_xor(s[q], off);
+        for (int q = 0; q < NR; ++q) {
+          const double xv = x[(int64_t)c * ldx + q];
+          s[q] += av * xv;
+          const unsigned long long xb = (unsigned long long)__double_as_longlong(fabs(xv));
+          xm[q] = xb > xm[q] ? xb : xm[q];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+          const unsigned long long o = __shfl_xor(xm[q], off);
+          xm[q] = o > xm[q] ? o : xm[q];
+        }
+      }
+    } else {
+      for (int c = lane; c < n; c += 64) {
+        const double av = arow[c];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) s[q] += av * x[(int64_t)c * ldx + q];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) s[q] += __shfl_xor(s[q], off);
+    }
   }
   if (lane == 0) {
 #pragma unroll
-    for (int q = 0; q < NR; ++q) r[(int64_t)row * ldr + q] = b[(int64_t)row * ldb + q] - s[q];
+    for (int q = 0; q < NR; ++q) {
+      unsigned long long m = 0ull;  // a missing row adds nothing to the maximum
+      if (has) {
+        const double rv = b[(int64_t)row * ldb + q] - s[q];
+        r[(int64_t)row * ldr + q] = rv;
+        m = (unsigned long long)__double_as_longlong(fabs(rv));
+      }
+      if (rbits) s_rm[wave][q] = m;
+      if (xmax) atomicMax(xbits + q, xm[q]);
+    }
+  }
+  if (rbits) {
+    // One atomic per workgroup and column: the largest of its four rows.
+    __syncthreads();
+    if (threadIdx.x < NR) {
+      const int q = threadIdx.x;
+      unsigned long long m = s_rm[0][q];
+#pragma unroll
+      for (int w = 1; w < 4; ++w) m = s_rm[w][q] > m ? s_rm[w][q] : m;
+      atomicMax(rbits + q, m);
+    }
   }
 }
 
@@ -1149,29 +1203,30 @@ __global__ void __launch_bounds__(256) splitk_sum_kernel(int n, int nrhs, int S,
 
 hipError_t launch_residual(lair_context *ctx, int n, int nrhs, const double *a, int64_t lda,
                            const double *x, int64_t ldx, const double *b, int64_t ldb, double *r,
-                           int64_t ldr) {
+                           int64_t ldr, unsigned long long *rbits, unsigned long long *xbits) {
   if (n <= 0 || nrhs <= 0) return hipSuccess;
   if (nrhs <= 4) {
     const int slot = prof_begin(ctx, LAIR_KCLASS_RESIDUAL);
     const unsigned grid = (n + 3) / 4;
     for (int q0 = 0; q0 < nrhs; q0 += 4) {
       const int nr = nrhs - q0 < 4 ? nrhs - q0 : 4;
+      unsigned long long *rb = rbits ? rbits + q0 : nullptr, *xb = xbits ? xbits + q0 : nullptr;
       switch (nr) {
         case 1:
           residual_kernel<1><<<grid, 256, 0, ctx->stream>>>(n, a, lda, x + q0, ldx, b + q0, ldb,
-                                                             r + q0, ldr);
+                                                             r + q0, ldr, rb, xb);
           break;
         case 2:
           residual_kernel<2><<<grid, 256, 0, ctx->stream>>>(n, a, lda, x + q0, ldx, b + q0, ldb,
-                                                             r + q0, ldr);
+                                                             r + q0, ldr, rb, xb);
           break;
         case 3:
           residual_kernel<3><<<grid, 256, 0, ctx->stream>>>(n, a, lda, x + q0, ldx, b + q0, ldb,
-                                                             r + q0, ldr);
+                                                             r + q0, ldr, rb, xb);
           break;
         default:
           residual_kernel<4><<<grid, 256, 0, ctx->stream>>>(n, a, lda, x + q0, ldx, b + q0, ldb,
-                                                             r + q0, ldr);
+                                                             r + q0, ldr, rb, xb);
       }
     }
     prof_end(ctx, slot, LAIR_KCLASS_RESIDUAL, 2.0 * n * (double)n * nrhs);
@@ -1195,6 +1250,9 @@ hipError_t launch_residual(lair_context *ctx, int n, int nrhs, const double *a, 
                                                                               ldr);
   prof_end(ctx, slot, LAIR_KCLASS_RESIDUAL, 2.0 * n * (double)n * nrhs,
            8.0 * ((double)n * n + (2.0 * S + 2.0) * n * nrhs));
+  if (rbits) e = launch_colmax(ctx, n, nrhs, r, ldr, rbits);
+  if (e == hipSuccess && xbits) e = launch_colmax(ctx, n, nrhs, x, ldx, xbits);
+  if (e != hipSuccess) return e;
   return hipGetLastError();
 }
 

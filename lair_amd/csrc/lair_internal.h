@@ -351,12 +351,16 @@ hipError_t launch_lu_extract(lair_context *ctx, int which, int m, int n, const d
 hipError_t launch_transpose_d(lair_context *ctx, int rows, int cols, const double *x, int64_t ldx,
                               double *y, int64_t ldy);
 // x += (double) d
+// x += d; words zero[0 .. nz) are cleared on the way (nullptr: none).
 hipError_t launch_axpy_s2d(lair_context *ctx, int rows, int cols, const float *d, int64_t ldd,
-                           double *x, int64_t ldx);
+                           double *x, int64_t ldx, unsigned long long *zero = nullptr, int nz = 0);
 // R = B - A X (FP64), nrhs columns.
+// r = b - A x; with rbits / xbits (cleared beforehand), also the column
+// maxima of |r| and |x| as ordered bit patterns.
 hipError_t launch_residual(lair_context *ctx, int n, int nrhs, const double *a, int64_t lda,
                            const double *x, int64_t ldx, const double *b, int64_t ldb, double *r,
-                           int64_t ldr);
+                           int64_t ldr, unsigned long long *rbits = nullptr,
+                           unsigned long long *xbits = nullptr);
 // per-column max |x| -> bits[col] (atomic max on uint64-ordered doubles).
 hipError_t launch_colmax(lair_context *ctx, int rows, int cols, const double *x, int64_t ldx,
                          unsigned long long *bits);
