@@ -1366,34 +1366,64 @@ lair_status lair_dgesv_mixed_device(lair_context *ctx, int64_t n64, int64_t nrhs
                                      false);
     return hip_status(e2, "trsv");
   };
-  st = lowprec_solve(b, ldb);
-  if (st != LAIR_OK) return st;
-  e = launch_copy_s2d(ctx, n, nrhs, d32, nrhs, x, ldx);
-  if (e != hipSuccess) return hip_status(e, "copy");
   bool converged = false;
   int it = 0;
   double be = INFINITY;
-  // The residual kernel folds in the norms of r and x; their words are
-  // cleared here once and then by each correction's x += d.
-  e = launch_fill_f64(ctx, reinterpret_cast<double *>(norms + 1), 2 * nrhs, 0.0);
-  if (e != hipSuccess) return hip_status(e, "fill");
-  for (;;) {
-    e = launch_residual(ctx, n, nrhs, a, lda, x, ldx, b, ldb, r, nrhs, norms + 1, norms + 1 + nrhs);
-    if (e != hipSuccess) return hip_status(e, "residual");
-    st = backward_error_read(ctx, nrhs, norms, &be);
-    if (st != LAIR_OK) return st;
-    if (it < 32) info->history[it] = be;
-    if (be <= tol) {
-      converged = true;
-      break;
+  if (n > kExactOrderMax && nrhs <= 4) {
+    // Each step is the gather of the right-hand side (which also clears the
+    // norm words), the forward solve, and one launch for the backward solve,
+    // x += d and the residual with its norms.  Step 0 solves with b from
+    // x = 0, so x = d exactly as a copy would give.
+    LAIR_HIP(ensure(ctx->ws_tmp2, sizeof(float) * (size_t)n * nrhs));
+    float *t = static_cast<float *>(ctx->ws_tmp2.ptr);
+    for (;;) {
+      const double *rhs = it == 0 ? b : r;
+      const int64_t ldr = it == 0 ? ldb : nrhs;
+      e = launch_permute_rows<double, float>(ctx, n, nrhs, aux.perm, rhs, ldr, t, nrhs, false, norms + 1,
+                                             2 * nrhs);
+      if (e != hipSuccess) return hip_status(e, "permute");
+      e = launch_trsv_dataflow<float>(ctx, a32, ld32, aux.linv, false, t, t, nrhs, n, nrhs, true, false);
+      if (e != hipSuccess) return hip_status(e, "trsv");
+      e = launch_trsv_ir(ctx, a32, ld32, aux.uinv, t, d32, n, nrhs, x, ldx, it == 0, a, lda, b, ldb, r, nrhs,
+                         norms + 1 + nrhs, norms + 1);
+      if (e != hipSuccess) return hip_status(e, "refinement step");
+      st = backward_error_read(ctx, nrhs, norms, &be);
+      if (st != LAIR_OK) return st;
+      if (it < 32) info->history[it] = be;
+      if (be <= tol) {
+        converged = true;
+        break;
+      }
+      if (it >= max_iter || !std::isfinite(be)) break;
+      ++it;
     }
-    if (it >= max_iter || !std::isfinite(be)) break;
-    // Correction d = LU^-1 r in the factorization precision, x += d in FP64.
-    st = lowprec_solve(r, nrhs);
+  } else {
+    st = lowprec_solve(b, ldb);
     if (st != LAIR_OK) return st;
-    e = launch_axpy_s2d(ctx, n, nrhs, d32, nrhs, x, ldx, norms + 1, 2 * nrhs);
-    if (e != hipSuccess) return hip_status(e, "axpy");
-    ++it;
+    e = launch_copy_s2d(ctx, n, nrhs, d32, nrhs, x, ldx);
+    if (e != hipSuccess) return hip_status(e, "copy");
+    // The residual kernel folds in the norms of r and x; their words are
+    // cleared here once and then by each correction's x += d.
+    e = launch_fill_f64(ctx, reinterpret_cast<double *>(norms + 1), 2 * nrhs, 0.0);
+    if (e != hipSuccess) return hip_status(e, "fill");
+    for (;;) {
+      e = launch_residual(ctx, n, nrhs, a, lda, x, ldx, b, ldb, r, nrhs, norms + 1, norms + 1 + nrhs);
+      if (e != hipSuccess) return hip_status(e, "residual");
+      st = backward_error_read(ctx, nrhs, norms, &be);
+      if (st != LAIR_OK) return st;
+      if (it < 32) info->history[it] = be;
+      if (be <= tol) {
+        converged = true;
+        break;
+      }
+      if (it >= max_iter || !std::isfinite(be)) break;
+      // Correction d = LU^-1 r in the factorization precision, x += d in FP64.
+      st = lowprec_solve(r, nrhs);
+      if (st != LAIR_OK) return st;
+      e = launch_axpy_s2d(ctx, n, nrhs, d32, nrhs, x, ldx, norms + 1, 2 * nrhs);
+      if (e != hipSuccess) return hip_status(e, "axpy");
+      ++it;
+    }
   }
   info->iterations = it;
   info->backward_error = be;

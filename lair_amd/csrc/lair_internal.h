@@ -268,7 +268,8 @@ hipError_t launch_perm_from_ipiv(lair_context *ctx, const int64_t *ipiv, int n, 
 hipError_t launch_iota(lair_context *ctx, int *p, int n);
 template <typename S, typename D>
 hipError_t launch_permute_rows(lair_context *ctx, int rows, int cols, const int *perm, const S *x,
-                               int64_t ldx, D *y, int64_t ldy, bool scatter);
+                               int64_t ldx, D *y, int64_t ldy, bool scatter,
+                               unsigned long long *zero = nullptr, int nz = 0);
 // B := L^-1 B (unit lower L, kb <= 256).
 template <typename T>
 hipError_t launch_trsm_llu(lair_context *ctx, int kb, int ncols, const T *L, int64_t ldl, T *B,
@@ -289,6 +290,16 @@ template <typename T>
 hipError_t launch_trsv_dataflow(lair_context *ctx, const T *lu, int64_t lda, const T *dinv,
                                 bool dinv_trans, const T *xin, T *x, int64_t ldx, int n, int nrhs,
                                 bool forward, bool trans);
+// One refinement step's second half in one launch (nrhs <= 4): the backward
+// solve d = U^-1 y in FP32 (y and d with leading dimension nrhs), x += d in
+// FP64 (x taken as zero when x_zero), r = b - A x in FP64 over the FP64 A,
+// and the column maxima of |x| and |r| into xbits / rbits (ordered bit
+// patterns that must be zero at launch).
+hipError_t launch_trsv_ir(lair_context *ctx, const float *lu, int64_t lda, const float *uinv,
+                          const float *y, float *d, int n, int nrhs, double *x, int64_t ldx,
+                          bool x_zero, const double *a, int64_t lda64, const double *b, int64_t ldb,
+                          double *r, int64_t ldr, unsigned long long *xbits,
+                          unsigned long long *rbits);
 
 // 128x128-tile MFMA trailing updates (gemm_kernels.hip).
 hipError_t launch_dgemm_tile(lair_context *ctx, int M, int N, int K, const double *A, int64_t lda,

@@ -18,6 +18,7 @@
 //    before waiting, so the sequential chain only carries one round trip, one
 //    tile product and one diagonal-inverse product per block (the
 //    substitutions of src/lapack/getrs.rs:24-36, blocked).
+#include <algorithm>
 #include <climits>
 
 #include "device_util.h"
@@ -284,8 +285,9 @@ hipError_t launch_iota(lair_context *ctx, int *p, int n) {
 // a null perm copies.
 template <typename S, typename D>
 __global__ void permute_rows_kernel(int rows, int cols, const int *perm, const S *x, int64_t ldx,
-                                    D *y, int64_t ldy, int scatter) {
+                                    D *y, int64_t ldy, int scatter, unsigned long long *zero, int nz) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (zero && idx < nz) zero[idx] = 0ull;
   if (idx >= (int64_t)rows * cols) return;
   const int i = (int)(idx / cols), q = (int)(idx % cols);
   const int pi = perm ? perm[i] : i;
@@ -297,23 +299,24 @@ __global__ void permute_rows_kernel(int rows, int cols, const int *perm, const S
 
 template <typename S, typename D>
 hipError_t launch_permute_rows(lair_context *ctx, int rows, int cols, const int *perm, const S *x,
-                               int64_t ldx, D *y, int64_t ldy, bool scatter) {
-  const int64_t total = (int64_t)rows * cols;
+                               int64_t ldx, D *y, int64_t ldy, bool scatter, unsigned long long *zero,
+                               int nz) {
+  const int64_t total = std::max<int64_t>((int64_t)rows * cols, zero ? nz : 0);
   if (total <= 0) return hipSuccess;
   permute_rows_kernel<S, D><<<(unsigned)((total + 255) / 256), 256, 0, ctx->stream>>>(
-      rows, cols, perm, x, ldx, y, ldy, scatter ? 1 : 0);
+      rows, cols, perm, x, ldx, y, ldy, scatter ? 1 : 0, zero, nz);
   return hipGetLastError();
 }
 
 template hipError_t launch_permute_rows<double, double>(lair_context *, int, int, const int *,
                                                         const double *, int64_t, double *, int64_t,
-                                                        bool);
+                                                        bool, unsigned long long *, int);
 template hipError_t launch_permute_rows<double, float>(lair_context *, int, int, const int *,
                                                        const double *, int64_t, float *, int64_t,
-                                                       bool);
+                                                       bool, unsigned long long *, int);
 template hipError_t launch_permute_rows<float, float>(lair_context *, int, int, const int *,
                                                       const float *, int64_t, float *, int64_t,
-                                                      bool);
+                                                      bool, unsigned long long *, int);
 
 // ------------------------------------------------------------ trsm_llu ---
 
@@ -686,6 +689,28 @@ struct TrsvArgs {
   int wait_limit;  // polls before a wait gives up (0: every wait gives up at once)
 };
 
+// The refinement step fused with the backward solve of the correction
+// d = U^-1 y (FP32): the workgroup that solves block i of d also forms
+// x_i + d_i in FP64, stores it and publishes it as tagged words (two per
+// value, tag = the solve's epoch); residual workgroups stream the FP64 A by
+// 64-row blocks and consume those values block by block in the order the
+// solve finishes them, so r = b - A (x + d) is complete a few microseconds
+// after the last block of d.  The column maxima of |x + d| and |r| fold into
+// xbits and rbits (ordered bit patterns, zero at launch).
+struct IrFuse {
+  double *x;
+  int64_t ldx;
+  int x_zero;                // 1: x starts at zero (x is not read)
+  const double *a;
+  int64_t lda;
+  const double *b;
+  int64_t ldb;
+  double *r;
+  int64_t ldr;
+  unsigned long long *xw;    // [n][nrhs][2] tagged words of x + d
+  unsigned long long *xbits, *rbits;
+};
+
 // Effective matrix E: forward & !trans -> L (strictly lower part of lu);
 // !forward & !trans -> U; forward & trans -> U^T; !forward & trans -> L^T.
 // Workgroup (block row i, column chunk ch) computes
@@ -693,8 +718,8 @@ struct TrsvArgs {
 // over the blocks j already solved (j < i forward, j > i backward).  Tiles
 // move global -> registers (16-byte lane-contiguous loads, issued before the
 // wait for the previous block) -> LDS -> row segments per thread.
-template <typename T, int BS, int RC>
-__global__ void __launch_bounds__(256) trsv_dataflow_kernel(TrsvArgs<T> p) {
+template <typename T, int BS, int RC, bool FUSE>
+__device__ __forceinline__ void trsv_block(const TrsvArgs<T> &p, int wid, const IrFuse &f) {
   constexpr int WPE = sizeof(T) / 4;  // 32-bit words per element
   constexpr int TPR = 256 / BS;   // threads per row in the product
   constexpr int SEG = BS / TPR;   // columns per thread
@@ -709,7 +734,6 @@ __global__ void __launch_bounds__(256) trsv_dataflow_kernel(TrsvArgs<T> p) {
   __shared__ int s_bail;
   const int t = threadIdx.x;
   const int r = t / TPR, h = t % TPR;
-  const int wid = blockIdx.x;
   const int ib = wid / p.nchunks, ch = wid % p.nchunks;
   const int i = p.forward ? ib : p.nblk - 1 - ib;
   const int r0 = i * BS;
@@ -773,6 +797,11 @@ __global__ void __launch_bounds__(256) trsv_dataflow_kernel(TrsvArgs<T> p) {
 #pragma unroll
   for (int q = 0; q < RC; ++q) acc[q] = T(0);
   if (t == 0) s_bail = 0;
+  // Fused refinement: this block's rows of x, read while the chain runs.
+  double xold[RC];
+#pragma unroll
+  for (int q = 0; q < RC; ++q)
+    xold[q] = (FUSE && !f.x_zero && h == 0 && rowok && q < ncols) ? f.x[(int64_t)gr * f.ldx + c0 + q] : 0.0;
 
   const int nb = p.forward ? i : p.nblk - 1 - i;  // number of dependency blocks
   auto issue = [&](int d, vec_t (&v)[NV]) {
@@ -882,6 +911,265 @@ __global__ void __launch_bounds__(256) trsv_dataflow_kernel(TrsvArgs<T> p) {
       }
     }
   }
+  if constexpr (FUSE) {
+    // x + d in FP64 (the chain's words above go first), published for the
+    // residual workgroups; the block's maxima of |x + d| by one atomic per
+    // column.
+    __shared__ unsigned long long s_xm[4][RC];
+    unsigned long long xm[RC];
+#pragma unroll
+    for (int q = 0; q < RC; ++q) {
+      xm[q] = 0ull;
+      if (h == 0 && rowok && q < ncols) {
+        const double xn = xold[q] + (double)out[q];
+        f.x[(int64_t)gr * f.ldx + c0 + q] = xn;
+        unsigned long long *w = f.xw + ((int64_t)gr * p.nrhs + c0 + q) * 2;
+        st_sc1(w, tagged(p.epoch, word_of<double>(xn, 0)));
+        st_sc1(w + 1, tagged(p.epoch, word_of<double>(xn, 1)));
+        xm[q] = (unsigned long long)__double_as_longlong(fabs(xn));
+      }
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) {
+        const unsigned long long o = __shfl_xor(xm[q], off);
+        xm[q] = o > xm[q] ? o : xm[q];
+      }
+    }
+    if ((t & 63) == 0) {
+#pragma unroll
+      for (int q = 0; q < RC; ++q) s_xm[t >> 6][q] = xm[q];
+    }
+    __syncthreads();
+    if (t < RC && t < ncols) {
+      unsigned long long m = s_xm[0][t];
+#pragma unroll
+      for (int w = 1; w < 4; ++w) m = s_xm[w][t] > m ? s_xm[w][t] : m;
+      atomicMax(f.xbits + c0 + t, m);
+    }
+  }
+}
+
+template <typename T, int BS, int RC>
+__global__ void __launch_bounds__(256) trsv_dataflow_kernel(TrsvArgs<T> p) {
+  trsv_block<T, BS, RC, false>(p, blockIdx.x, IrFuse{});
+}
+
+// Residual workgroup rw of the fused refinement step: rows
+// 3 RPW rw .. 3 RPW (rw + 1) - 1 of r = b - A (x + d).  Wave 0 only waits for
+// the values: per column block (in the backward solve's order, last block
+// first) its lane 0 polls the block's first word, then every lane reads the
+// words of its two columns and the values go to LDS (by block parity).
+// Waves 1..3 hold RPW rows each, two columns per lane, and keep the next
+// block's A loads in flight across the workgroup barrier
+// (which waits for LDS only), so HBM latency overlaps the chain.  The
+// per-lane partial sums of a wave's rows are reduced by a butterfly that
+// halves the rows at each step (the top lane bits select the row).
+template <int RC>
+__device__ __forceinline__ void ir_residual_block(const TrsvArgs<float> &p, const IrFuse &f, int rw) {
+  constexpr int BS = TrsvBlock<float>::BS;  // 2 columns per lane
+  constexpr int RPW = RC == 1 ? 16 : 8;      // rows per worker wave
+  constexpr int LOGR = RPW == 16 ? 4 : 3;
+  static_assert(BS == 128, "two columns per lane");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double *sx = reinterpret_cast<double *>(smem_raw);  // [2][BS][RC]
+  __shared__ unsigned long long s_rm[4][RC];
+  __shared__ int s_bail[2];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n = p.n, nq = p.nrhs, nblk = p.nblk;
+  const int row0 = rw * 3 * RPW + (wave - 1) * RPW;
+
+  // Worker waves: A's values of block d, columns cb + lane and cb + 64 + lane
+  // of the wave's rows (each load instruction reads 512 contiguous bytes).
+  // Rows and columns past the end are clamped to the last one, so the loads
+  // carry no branch (the compiler can then wait for one block's loads while
+  // the next block's are in flight); the products mask those columns.
+  auto load_a = [&](int d, double (&a0)[RPW], double (&a1)[RPW]) {
+    const int cb = (nblk - 1 - d) * BS;
+    const int c0 = min(cb + lane, n - 1), c1 = min(cb + 64 + lane, n - 1);
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) {
+      const double *ap = f.a + (int64_t)min(row0 + i, n - 1) * f.lda;
+      a0[i] = ap[c0];
+      a1[i] = ap[c1];
+    }
+  };
+  // Wave 0: block d's values of x + d into LDS; returns false if a wait gave up.
+  auto wait_x = [&](int d) -> bool {
+    const int cb = (nblk - 1 - d) * BS;
+    double *dst = sx + (d & 1) * BS * RC;
+    bool bail = false;
+    int spins = 0;
+    auto ready = [&](const unsigned long long *w, double *v) -> bool {
+      const unsigned long long w0 = ld_sc1(w), w1 = ld_sc1(w + 1);
+      if ((uint32_t)(w0 >> 32) != p.epoch || (uint32_t)(w1 >> 32) != p.epoch) return false;
+      const uint32_t part[2] = {(uint32_t)w0, (uint32_t)w1};
+      *v = from_words<double>(part);
+      return true;
+    };
+    auto spin = [&]() -> bool {  // false once the wait gives up
+      if (spins >= p.wait_limit) return false;
+      if ((++spins & 63) == 0 && ld_sc1(p.err) != 0) return false;
+      __builtin_amdgcn_s_sleep(2);
+      return true;
+    };
+    // Every lane reads the words of its two columns (one round trip when the
+    // block is already solved); while some are missing, lane 0 polls the
+    // first missing one and then the lanes read theirs again.
+    double v[2][RC];
+    bool have[2][RC];
+#pragma unroll
+    for (int k2 = 0; k2 < 2; ++k2)
+#pragma unroll
+      for (int q = 0; q < RC; ++q) {
+        v[k2][q] = 0.0;
+        have[k2][q] = !(cb + 64 * k2 + lane < n && q < nq);
+      }
+    for (;;) {
+      bool all = true;
+#pragma unroll
+      for (int k2 = 0; k2 < 2; ++k2)
+#pragma unroll
+        for (int q = 0; q < RC; ++q)
+          if (!have[k2][q]) {
+            have[k2][q] = ready(f.xw + ((int64_t)(cb + 64 * k2 + lane) * nq + q) * 2, &v[k2][q]);
+            all = all && have[k2][q];
+          }
+      const unsigned long long missing = __ballot(!all);
+      if (!missing) break;
+      // The first missing word of the first lane that lacks one.
+      const int ml = __ffsll((long long)missing) - 1;
+      int mw = 0;
+#pragma unroll
+      for (int k2 = 1; k2 >= 0; --k2)
+#pragma unroll
+        for (int q = RC - 1; q >= 0; --q)
+          if (!have[k2][q]) mw = (cb + 64 * k2 + lane) * nq + q;
+      mw = __shfl(mw, ml);
+      if (lane == 0) {
+        double tmp;
+        while (!ready(f.xw + (int64_t)mw * 2, &tmp))
+          if (!spin()) {
+            bail = true;
+            break;
+          }
+      }
+      if (__shfl((int)bail, 0)) {
+        bail = true;
+        break;
+      }
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < 2; ++k2)
+#pragma unroll
+      for (int q = 0; q < RC; ++q) dst[(64 * k2 + lane) * RC + q] = bail ? 0.0 : v[k2][q];
+    return !__any(bail);
+  };
+
+  if (threadIdx.x == 0) s_bail[0] = s_bail[1] = 0;
+  double acc[RPW][RC];
+#pragma unroll
+  for (int i = 0; i < RPW; ++i)
+#pragma unroll
+    for (int q = 0; q < RC; ++q) acc[i][q] = 0.0;
+  double ba0[RPW], ba1[RPW], bb0[RPW], bb1[RPW];
+  if (wave > 0) load_a(0, ba0, ba1);
+  // One block: wave 0 waits for its values while the workers issue the next
+  // block's loads; after the barrier the workers multiply.  The bail word
+  // alternates with the LDS buffer, so every wave leaves at the same block.
+  auto step = [&](int d, double (&c0)[RPW], double (&c1)[RPW], double (&n0)[RPW], double (&n1)[RPW]) -> bool {
+    if (wave == 0) {
+      const bool ok = wait_x(d);
+      if (lane == 0) s_bail[d & 1] = ok ? 0 : 1;
+    } else if (d + 1 < nblk) {
+      load_a(d + 1, n0, n1);
+    }
+    __syncthreads();
+    if (s_bail[d & 1]) return false;
+    if (wave > 0) {
+      const double *xs = sx + (d & 1) * BS * RC + lane * RC;
+      const int cb = (nblk - 1 - d) * BS;
+      const bool ok0 = cb + lane < n, ok1 = cb + 64 + lane < n;
+#pragma unroll
+      for (int q = 0; q < RC; ++q) {
+        const double x0 = xs[q], x1 = xs[64 * RC + q];
+#pragma unroll
+        for (int i = 0; i < RPW; ++i)
+          acc[i][q] += (ok0 ? c0[i] : 0.0) * x0 + (ok1 ? c1[i] : 0.0) * x1;
+      }
+    }
+    return true;
+  };
+  bool ok = true;
+  for (int d = 0; d < nblk && ok; d += 2) {
+    ok = step(d, ba0, ba1, bb0, bb1);
+    if (ok && d + 1 < nblk) ok = step(d + 1, bb0, bb1, ba0, ba1);
+  }
+  if (!ok && threadIdx.x == 0) atomic_or_agent(p.err, kErrWait);
+  unsigned long long rm[RC];
+#pragma unroll
+  for (int q = 0; q < RC; ++q) rm[q] = 0ull;
+  if (wave > 0) {
+#pragma unroll
+    for (int q = 0; q < RC; ++q) {
+      // Butterfly over lane bits 5, 4, ...: a lane keeps half of its rows,
+      // summed with its partner's; then the lanes that share a row sum.
+#pragma unroll
+      for (int h = RPW / 2, o = 32; h >= 1; h >>= 1, o >>= 1) {
+        const bool up = (lane & o) != 0;
+#pragma unroll
+        for (int m = 0; m < h; ++m) {
+          const double send = up ? acc[m][q] : acc[m + h][q];
+          const double keep = up ? acc[m + h][q] : acc[m][q];
+          acc[m][q] = keep + __shfl_xor(send, o);
+        }
+      }
+#pragma unroll
+      for (int o = 32 >> LOGR; o >= 1; o >>= 1) acc[0][q] += __shfl_xor(acc[0][q], o);
+    }
+    int ri = 0;
+#pragma unroll
+    for (int s = 0; s < LOGR; ++s) ri += ((lane >> (5 - s)) & 1) * ((RPW / 2) >> s);
+    const int row = row0 + ri;
+    const bool owner = (lane & ((64 / RPW) - 1)) == 0;
+#pragma unroll
+    for (int q = 0; q < RC; ++q) {
+      if (ok && owner && row < n && q < nq) {
+        const double rv = f.b[(int64_t)row * f.ldb + q] - acc[0][q];
+        f.r[(int64_t)row * f.ldr + q] = rv;
+        rm[q] = (unsigned long long)__double_as_longlong(fabs(rv));
+      }
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) {
+        const unsigned long long o = __shfl_xor(rm[q], off);
+        rm[q] = o > rm[q] ? o : rm[q];
+      }
+    }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < RC; ++q) s_rm[wave][q] = rm[q];
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < RC && (int)threadIdx.x < nq) {
+    const int q = threadIdx.x;
+    unsigned long long m = s_rm[0][q];
+#pragma unroll
+    for (int w = 1; w < 4; ++w) m = s_rm[w][q] > m ? s_rm[w][q] : m;
+    atomicMax(f.rbits + q, m);
+  }
+}
+
+// The fused refinement step: workgroups 0 .. nblk - 1 run the backward
+// solve (and x += d), the rest the residual.  The solve's workgroups come
+// first in dispatch order and wait only on each other, the residual's wait
+// only on the solve, so every wait ends whichever workgroups are resident.
+template <int RC>
+__global__ void __launch_bounds__(256) trsv_ir_kernel(TrsvArgs<float> p, IrFuse f) {
+  constexpr int BS = TrsvBlock<float>::BS;
+  const int ntri = p.nblk * p.nchunks;
+  if ((int)blockIdx.x < ntri)
+    trsv_block<float, BS, RC, true>(p, blockIdx.x, f);
+  else
+    ir_residual_block<RC>(p, f, blockIdx.x - ntri);
 }
 
 // Many right-hand sides in FP32 (nrhs >= 16, not transposed): the same
@@ -1046,9 +1334,9 @@ __global__ void __launch_bounds__(256, 2) trsm_mfma_kernel(TrsvArgs<float> p) {
 }
 
 template <typename T>
-hipError_t launch_trsv_dataflow(lair_context *ctx, const T *lu, int64_t lda, const T *dinv,
-                                bool dinv_trans, const T *xin, T *x, int64_t ldx, int n, int nrhs,
-                                bool forward, bool trans) {
+static hipError_t trsv_launch(lair_context *ctx, const T *lu, int64_t lda, const T *dinv,
+                              bool dinv_trans, const T *xin, T *x, int64_t ldx, int n, int nrhs,
+                              bool forward, bool trans, IrFuse *fz) {
   if (n <= 0 || nrhs <= 0) return hipSuccess;
   constexpr int BS = TrsvBlock<T>::BS;
   // Right-hand sides per workgroup: 4 columns keep each chain link short
@@ -1064,7 +1352,8 @@ hipError_t launch_trsv_dataflow(lair_context *ctx, const T *lu, int64_t lda, con
   // caller-provided buffer (xt_override) was zeroed by the caller.
   unsigned long long *xt = ctx->xt_override;
   if (!xt) {
-    const size_t xt_bytes = sizeof(unsigned long long) * (size_t)n * nrhs * (sizeof(T) / 4);
+    // The fused refinement step also needs two words per value of x + d.
+    const size_t xt_bytes = sizeof(unsigned long long) * (size_t)n * nrhs * (sizeof(T) / 4 + (fz ? 2 : 0));
     const bool grow = ctx->ws_flags.bytes < xt_bytes;
     hipError_t e = ensure(ctx->ws_flags, xt_bytes);
     if (e != hipSuccess) return e;
@@ -1103,6 +1392,25 @@ hipError_t launch_trsv_dataflow(lair_context *ctx, const T *lu, int64_t lda, con
   static bool once = (allow_large_lds(trsv_dataflow_kernel<T, BS, 1>),
                       allow_large_lds(trsv_dataflow_kernel<T, BS, 4>), true);
   (void)once;
+  if constexpr (std::is_same<T, float>::value) {
+    if (fz) {
+      static bool once_f = (allow_large_lds(trsv_ir_kernel<1>), allow_large_lds(trsv_ir_kernel<4>), true);
+      (void)once_f;
+      fz->xw = xt + (size_t)n * nrhs;
+      // Residual workgroups: 48 rows each for one right-hand side, else 24.
+      const int rrows = rc == 1 ? 48 : 24;
+      const unsigned gridf = grid + (unsigned)((n + rrows - 1) / rrows);
+      const int slot = prof_begin(ctx, LAIR_KCLASS_GETRS);
+      if (rc == 1)
+        trsv_ir_kernel<1><<<gridf, 256, shm, ctx->stream>>>(a, *fz);
+      else
+        trsv_ir_kernel<4><<<gridf, 256, shm, ctx->stream>>>(a, *fz);
+      // The solve's bytes plus one pass over the FP64 A.
+      prof_end(ctx, slot, LAIR_KCLASS_GETRS, (double)n * n * nrhs * 3,
+               sizeof(T) * 0.5 * (double)n * n + 8.0 * n * n + 32.0 * n * nrhs);
+      return hipGetLastError();
+    }
+  }
   const int slot = prof_begin(ctx, LAIR_KCLASS_GETRS);
   if constexpr (std::is_same<T, float>::value) {
     if (mfma) {
@@ -1121,6 +1429,34 @@ hipError_t launch_trsv_dataflow(lair_context *ctx, const T *lu, int64_t lda, con
   prof_end(ctx, slot, LAIR_KCLASS_GETRS, (double)n * n * nrhs,
            sizeof(T) * (0.5 * (double)n * n + 2.0 * n * nrhs));
   return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_trsv_dataflow(lair_context *ctx, const T *lu, int64_t lda, const T *dinv,
+                                bool dinv_trans, const T *xin, T *x, int64_t ldx, int n, int nrhs,
+                                bool forward, bool trans) {
+  return trsv_launch<T>(ctx, lu, lda, dinv, dinv_trans, xin, x, ldx, n, nrhs, forward, trans, nullptr);
+}
+
+hipError_t launch_trsv_ir(lair_context *ctx, const float *lu, int64_t lda, const float *uinv,
+                          const float *y, float *d, int n, int nrhs, double *x, int64_t ldx,
+                          bool x_zero, const double *a, int64_t lda64, const double *b, int64_t ldb,
+                          double *r, int64_t ldr, unsigned long long *xbits,
+                          unsigned long long *rbits) {
+  if (nrhs > 4) return hipErrorInvalidValue;
+  IrFuse fz{};
+  fz.x = x;
+  fz.ldx = ldx;
+  fz.x_zero = x_zero ? 1 : 0;
+  fz.a = a;
+  fz.lda = lda64;
+  fz.b = b;
+  fz.ldb = ldb;
+  fz.r = r;
+  fz.ldr = ldr;
+  fz.xbits = xbits;
+  fz.rbits = rbits;
+  return trsv_launch<float>(ctx, lu, lda, uinv, false, y, d, nrhs, n, nrhs, false, false, &fz);
 }
 
 template hipError_t launch_trsv_dataflow<double>(lair_context *, const double *, int64_t,

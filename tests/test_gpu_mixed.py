@@ -32,9 +32,9 @@ def host_berr(a, x, b):
     return float((np.abs(r).max(0) / (np.abs(a).sum(1).max() * np.abs(x).max(0))).max())
 
 
-def run_mixed(ctx, a, b, prec, max_iter=20):
+def run_mixed(ctx, a, b, prec, max_iter=20, pad=64):
     n, nrhs = b.shape
-    ld = n + 64
+    ld = n + pad
     da = L.DeviceArray.from_host(a, ld=ld)
     db = L.DeviceArray.from_host(b)
     dx = L.DeviceArray(n, nrhs)
@@ -64,6 +64,27 @@ def test_mixed_converges_to_fp64_backward_error(ctx, prec, n, nrhs):
     assert be <= n * EPS
     # the device's own estimate agrees with the host check
     assert abs(info.backward_error - be) <= 0.5 * be + 1e-18
+
+
+# Orders that are not a multiple of the 128-row solve block, odd leading
+# dimensions (the residual's A rows lose their 16-byte alignment) and 1 to 3
+# right-hand sides: the refinement step that fuses the backward solve, x += d
+# and the residual covers them all.
+@pytest.mark.parametrize("prec", [_capi.PREC_FP32, _capi.PREC_BF16X6])
+@pytest.mark.parametrize("n,nrhs,pad", [(333, 2, 0), (1001, 3, 2), (640, 1, 1), (129, 4, 0)])
+def test_mixed_ragged_shapes(ctx, prec, n, nrhs, pad):
+    a = kappa_matrix(41 + n, n)
+    b = inputs.gaussian_matrix(43 + n, n, nrhs)
+    st, info, x = run_mixed(ctx, a, b, prec, pad=pad)
+    assert st == _capi.LAIR_OK, _capi.last_error()
+    assert info.converged == 1 and info.fell_back == 0
+    be = host_berr(a, x, b)
+    assert be <= n * EPS
+    assert abs(info.backward_error - be) <= 0.5 * be + 1e-18
+    # The history is the backward error after each step, ending at the
+    # reported one.
+    hist = [info.history[i] for i in range(info.iterations + 1)]
+    assert hist[-1] == info.backward_error and all(h > n * EPS for h in hist[:-1])
 
 
 def test_single_bf16_product_falls_back_to_fp64(ctx):
