@@ -7,6 +7,7 @@
 //   lapack::laswp   src/lapack/laswp.rs:11-40   -> lair_dlaswp
 //   equation::solve src/equation.rs:32-60       -> lair_dsolve
 #include <algorithm>
+#include <chrono>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -536,14 +537,9 @@ static lair_status backward_error_from(lair_context *ctx, int n, int nrhs, const
   return backward_error_read(ctx, nrhs, norms, berr);
 }
 
-// The worst column's ||r||/(||A|| ||x||) from norms = [||A||, ||r_c||..., ||x_c||...]
-// (bit patterns on the device); synchronizes the stream.
-static lair_status backward_error_read(lair_context *ctx, int nrhs, const unsigned long long *norms,
-                                       double *berr) {
-  std::vector<unsigned long long> h(1 + 2 * (size_t)nrhs);
-  LAIR_HIP(hipMemcpyAsync(h.data(), norms, h.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
-  lair_status st = sync_status(ctx);
-  if (st != LAIR_OK) return st;
+// The worst column's ||r||/(||A|| ||x||) from the words [||A||, ||r_c||...,
+// ||x_c||...] (ordered bit patterns) read back to the host.
+static double berr_from_words(const unsigned long long *h, int nrhs) {
   const double na = bits_to_double(h[0]);
   double worst = 0.0;
   for (int c = 0; c < nrhs; ++c) {
@@ -558,7 +554,47 @@ static lair_status backward_error_read(lair_context *ctx, int nrhs, const unsign
       be = nr / (na * nx);
     if (!(be <= worst)) worst = be;  // NaN propagates
   }
-  *berr = worst;
+  return worst;
+}
+
+// The same from norms on the device; synchronizes the stream.
+static lair_status backward_error_read(lair_context *ctx, int nrhs, const unsigned long long *norms,
+                                       double *berr) {
+  std::vector<unsigned long long> h(1 + 2 * (size_t)nrhs);
+  LAIR_HIP(hipMemcpyAsync(h.data(), norms, h.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+  lair_status st = sync_status(ctx);
+  if (st != LAIR_OK) return st;
+  *berr = berr_from_words(h.data(), nrhs);
+  return LAIR_OK;
+}
+
+// The same without a stream synchronization: a one-wave kernel copies the
+// norm words and the status word into pinned host memory and then writes a
+// fresh sequence number (system-scope release); the host polls that number.
+// After kPublishSpinMs without it, the host synchronizes the stream instead
+// (which also reports a failed launch).  A set status word is reported (and
+// reset) by sync_status.
+constexpr double kPublishSpinMs = 20.0;
+static lair_status backward_error_published(lair_context *ctx, int nrhs, const unsigned long long *norms,
+                                            double *berr) {
+  if (!ctx->pub_host) return backward_error_read(ctx, nrhs, norms, berr);
+  const int nw = 1 + 2 * nrhs;
+  const unsigned long long seq = ++ctx->pub_seq;
+  LAIR_HIP(launch_publish_words(ctx, norms, nw, seq));
+  const auto t0 = std::chrono::steady_clock::now();
+  while (__atomic_load_n(&ctx->pub_host[0], __ATOMIC_ACQUIRE) != seq) {
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (ms > kPublishSpinMs) {
+      LAIR_HIP(hipStreamSynchronize(ctx->stream));
+      if (__atomic_load_n(&ctx->pub_host[0], __ATOMIC_ACQUIRE) != seq) {
+        set_error("published norms did not arrive");
+        return LAIR_ERR_DEVICE;
+      }
+      break;
+    }
+  }
+  if (ctx->pub_host[1] != 0) return sync_status(ctx);
+  *berr = berr_from_words(ctx->pub_host + 2, nrhs);
   return LAIR_OK;
 }
 
@@ -854,6 +890,17 @@ lair_status lair_context_create(lair_context **out) {
   LAIR_HIP(hipEventCreateWithFlags(&ctx->ev_panel, hipEventDisableTiming));
   LAIR_HIP(hipEventCreateWithFlags(&ctx->ev_bulk, hipEventDisableTiming));
   LAIR_HIP(hipMalloc(&ctx->norms, sizeof(double) * 4096));
+  // Pinned, coherent host words for the refinement's published norms; without
+  // them the norms are read back with a stream synchronization.
+  if (hipHostMalloc(reinterpret_cast<void **>(&ctx->pub_host), sizeof(unsigned long long) * (2 + 4096),
+                    hipHostMallocCoherent | hipHostMallocMapped) == hipSuccess) {
+    std::memset(ctx->pub_host, 0, sizeof(unsigned long long) * (2 + 4096));
+    if (hipHostGetDevicePointer(reinterpret_cast<void **>(&ctx->pub_dev), ctx->pub_host, 0) != hipSuccess) {
+      (void)hipHostFree(ctx->pub_host);
+      ctx->pub_host = ctx->pub_dev = nullptr;
+    }
+  }
+  (void)hipGetLastError();
   LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->rec), 4 * kMaxPanelWG * 64, 0u));
   LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->drec), 4 * 64, 0u));
   LAIR_HIP(launch_fill_u32(ctx.get(), reinterpret_cast<uint32_t *>(ctx->crec), 2 * kCaluWords, 0u));
@@ -881,6 +928,7 @@ lair_status lair_context_destroy(lair_context *ctx) {
   hipFree(ctx->rp_rowd);
   hipFree(ctx->sing);
   hipFree(ctx->norms);
+  if (ctx->pub_host) hipHostFree(ctx->pub_host);
   hipFree(ctx->swap_rows);
   hipFree(ctx->swap_src);
   hipFree(ctx->swap_count);
@@ -1387,7 +1435,7 @@ lair_status lair_dgesv_mixed_device(lair_context *ctx, int64_t n64, int64_t nrhs
       e = launch_trsv_ir(ctx, a32, ld32, aux.uinv, t, d32, n, nrhs, x, ldx, it == 0, a, lda, b, ldb, r, nrhs,
                          norms + 1 + nrhs, norms + 1);
       if (e != hipSuccess) return hip_status(e, "refinement step");
-      st = backward_error_read(ctx, nrhs, norms, &be);
+      st = backward_error_published(ctx, nrhs, norms, &be);
       if (st != LAIR_OK) return st;
       if (it < 32) info->history[it] = be;
       if (be <= tol) {

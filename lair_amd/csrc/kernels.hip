@@ -958,6 +958,26 @@ hipError_t launch_fill_f64(lair_context *ctx, double *p, int64_t n, double v) {
   return hipGetLastError();
 }
 
+// One wave: the words and the status word to host memory, every lane's
+// stores completed and written back (system-scope release fence, which waits
+// for the whole wave's stores), then the sequence number.
+__global__ void __launch_bounds__(64) publish_words_kernel(const unsigned long long *src, int nw,
+                                                           const int32_t *err, unsigned long long *dst,
+                                                           unsigned long long seq) {
+  const int l = threadIdx.x;
+  for (int i = l; i < nw; i += 64) dst[2 + i] = src[i];
+  if (l == 0) dst[1] = (unsigned long long)(uint32_t)*err;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+  if (l == 0) __hip_atomic_store(dst, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+hipError_t launch_publish_words(lair_context *ctx, const unsigned long long *src, int nw,
+                                unsigned long long seq) {
+  if (!ctx->pub_dev || nw < 0 || nw > 4096) return hipErrorInvalidValue;
+  publish_words_kernel<<<1, 64, 0, ctx->stream>>>(src, nw, ctx->err, ctx->pub_dev, seq);
+  return hipGetLastError();
+}
+
 __device__ __forceinline__ void atomic_max_nonneg(unsigned long long *bits, double v) {
   // Non-negative doubles order like their bit patterns; NaN maps above +inf.
   atomicMax(bits, (unsigned long long)__double_as_longlong(v));

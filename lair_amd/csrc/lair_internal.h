@@ -134,6 +134,10 @@ struct lair_context {
   int rp_max_wg = lair::kMaxPanelWG;
   int64_t *sing = nullptr;     // last zero-pivot column (device)
   double *norms = nullptr;     // scratch for norm reductions (uint64-ordered doubles)
+  // Pinned host words [sequence, status, words...] written by
+  // launch_publish_words (device address pub_dev), and the last sequence.
+  unsigned long long *pub_host = nullptr, *pub_dev = nullptr;
+  unsigned long long pub_seq = 0;
 
   // Row-interchange map of the current panel (device); the *_alt set belongs
   // to the look-ahead stream (swapped in by lair::SideStream).
@@ -336,6 +340,10 @@ hipError_t launch_batched_gesv(lair_context *ctx, const BatchLaunch &bl, T *a, T
 size_t batched_lds_bytes_for(int elem_size, int nb, int lds_max_elems);
 void batched_prepare();
 
+// Copies nw words and the status word to ctx->pub_dev[2..] and [1], then
+// writes seq to ctx->pub_dev[0] (system-scope release) for the host to poll.
+hipError_t launch_publish_words(lair_context *ctx, const unsigned long long *src, int nw,
+                                unsigned long long seq);
 hipError_t launch_fill_i64(lair_context *ctx, int64_t *p, int64_t n, int64_t v);
 hipError_t launch_fill_u32(lair_context *ctx, uint32_t *p, int64_t n, uint32_t v);
 hipError_t launch_fill_f64(lair_context *ctx, double *p, int64_t n, double v);
