@@ -984,15 +984,48 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned long long *bits, doub
 }
 
 // One wave per row: y = (float) a, and ||row||_1 folded into the norm.
+// When both rows are 16-byte aligned (vec), a lane takes 4 consecutive
+// elements per step (two 16-byte loads, one 16-byte store), two steps in
+// flight; the tail and unaligned rows go element by element.
 __global__ void convert_d2s_norm_kernel(int n, int ncols, const double *a, int64_t lda, float *a32,
-                                        int64_t lda32, unsigned long long *norm_bits) {
+                                        int64_t lda32, unsigned long long *norm_bits, int vec) {
   const int lane = threadIdx.x & 63;
   const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (row >= n) return;
   const double *src = a + (int64_t)row * lda;
-  float *dst = a32 + (int64_t)row * lda32;
+  float *dst = a32 ? a32 + (int64_t)row * lda32 : nullptr;
   double s = 0.0;
-  for (int c = lane; c < ncols; c += 64) {
+  int c0 = 0;
+  if (vec) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const int nv = ncols / 512 * 512;  // whole steps of 2 x 64 lanes x 4 elements
+    for (int c = 4 * lane; c < nv; c += 512) {
+      d2 x[4];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        x[2 * u] = *reinterpret_cast<const d2 *>(src + c + 256 * u);
+        x[2 * u + 1] = *reinterpret_cast<const d2 *>(src + c + 256 * u + 2);
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        s += fabs(x[2 * u].x);
+        s += fabs(x[2 * u].y);
+        s += fabs(x[2 * u + 1].x);
+        s += fabs(x[2 * u + 1].y);
+        if (dst) {
+          f4 y;
+          y.x = (float)x[2 * u].x;
+          y.y = (float)x[2 * u].y;
+          y.z = (float)x[2 * u + 1].x;
+          y.w = (float)x[2 * u + 1].y;
+          *reinterpret_cast<f4 *>(dst + c + 256 * u) = y;
+        }
+      }
+    }
+    c0 = nv;
+  }
+  for (int c = c0 + lane; c < ncols; c += 64) {
     const double x = src[c];
     s += fabs(x);
     if (dst) dst[c] = (float)x;
@@ -1007,8 +1040,10 @@ hipError_t launch_convert_d2s_norm(lair_context *ctx, int n, int ncols, const do
                                    unsigned long long *norm_bits) {
   if (n <= 0) return hipSuccess;
   const int slot = prof_begin(ctx, LAIR_KCLASS_CONVERT);
+  const bool vec = reinterpret_cast<uintptr_t>(a) % 16 == 0 && lda % 2 == 0 &&
+                   (!a32 || (reinterpret_cast<uintptr_t>(a32) % 16 == 0 && lda32 % 4 == 0));
   convert_d2s_norm_kernel<<<(n + 3) / 4, 256, 0, ctx->stream>>>(n, ncols, a, lda, a32, lda32,
-                                                                norm_bits);
+                                                                norm_bits, vec ? 1 : 0);
   prof_end(ctx, slot, LAIR_KCLASS_CONVERT, 0.0);
   return hipGetLastError();
 }
