@@ -693,9 +693,9 @@ struct TrsvArgs {
 // d = U^-1 y (FP32): the workgroup that solves block i of d also forms
 // x_i + d_i in FP64, stores it and publishes it as tagged words (two per
 // value, tag = the solve's epoch); residual workgroups stream the FP64 A by
-// 64-row blocks and consume those values block by block in the order the
-// solve finishes them, so r = b - A (x + d) is complete a few microseconds
-// after the last block of d.  The column maxima of |x + d| and |r| fold into
+// row blocks (48 rows for one right-hand side, else 24) and consume those
+// values column block by column block in the order the solve finishes them,
+// so the residual's pass over A overlaps the solve's chain.  The column maxima of |x + d| and |r| fold into
 // xbits and rbits (ordered bit patterns, zero at launch).
 struct IrFuse {
   double *x;
@@ -956,8 +956,8 @@ __global__ void __launch_bounds__(256) trsv_dataflow_kernel(TrsvArgs<T> p) {
 // Residual workgroup rw of the fused refinement step: rows
 // 3 RPW rw .. 3 RPW (rw + 1) - 1 of r = b - A (x + d).  Wave 0 only waits for
 // the values: per column block (in the backward solve's order, last block
-// first) its lane 0 polls the block's first word, then every lane reads the
-// words of its two columns and the values go to LDS (by block parity).
+// first) every lane reads the words of its two columns, lane 0 alone polls
+// while some are missing, and the values go to LDS (by block parity).
 // Waves 1..3 hold RPW rows each, two columns per lane, and keep the next
 // block's A loads in flight across the workgroup barrier
 // (which waits for LDS only), so HBM latency overlaps the chain.  The
