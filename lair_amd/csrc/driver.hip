@@ -1417,7 +1417,12 @@ lair_status lair_dgesv_mixed_device(lair_context *ctx, int64_t n64, int64_t nrhs
   bool converged = false;
   int it = 0;
   double be = INFINITY;
-  if (n > kExactOrderMax && nrhs <= 4) {
+  // The fused step's workgroups (128 rows of the solve each, and 48 rows of
+  // the residual for one right-hand side, 24 for up to four) must all be
+  // resident at two per CU, or the residual's later workgroups would start
+  // only after the solve.
+  const int64_t fused_wgs = (n + 127) / 128 + (nrhs == 1 ? (n + 47) / 48 : (n + 23) / 24);
+  if (n > kExactOrderMax && nrhs <= 4 && fused_wgs <= 2 * (int64_t)ctx->num_cu) {
     // Each step is the gather of the right-hand side (which also clears the
     // norm words), the forward solve, and one launch for the backward solve,
     // x += d and the residual with its norms.  Step 0 solves with b from
@@ -1457,7 +1462,7 @@ lair_status lair_dgesv_mixed_device(lair_context *ctx, int64_t n64, int64_t nrhs
     for (;;) {
       e = launch_residual(ctx, n, nrhs, a, lda, x, ldx, b, ldb, r, nrhs, norms + 1, norms + 1 + nrhs);
       if (e != hipSuccess) return hip_status(e, "residual");
-      st = backward_error_read(ctx, nrhs, norms, &be);
+      st = backward_error_published(ctx, nrhs, norms, &be);
       if (st != LAIR_OK) return st;
       if (it < 32) info->history[it] = be;
       if (be <= tol) {
