@@ -620,31 +620,32 @@ __global__ void __launch_bounds__(BS)
     X[r * LD + c] = T(0);
   }
   __syncthreads();
+  // Column j of the inverse by substitution; each entry's dot product runs
+  // over four accumulators, eight terms per step with their LDS loads issued
+  // together (the loads, not the multiply-adds, set the time).
+  auto dot = [&](int i, int p0, int p1) -> T {  // sum over p in [p0, p1) of M[i][p] X[p][j]
+    T a[4] = {T(0), T(0), T(0), T(0)};
+    int p = p0;
+    for (; p + 8 <= p1; p += 8) {
+      T m[8], x[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        m[q] = M[i * LD + p + q];
+        x[q] = X[(p + q) * LD + j];
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) a[q & 3] += m[q] * x[q];
+    }
+    for (; p < p1; ++p) a[p & 3] += M[i * LD + p] * X[p * LD + j];
+    return (a[0] + a[1]) + (a[2] + a[3]);
+  };
   if (j < nb) {
     if (!upper) {
       X[j * LD + j] = T(1);
-      for (int i = j + 1; i < nb; ++i) {
-        T a0 = T(0), a1 = T(0);
-        int p = j;
-        for (; p + 1 < i; p += 2) {
-          a0 -= M[i * LD + p] * X[p * LD + j];
-          a1 -= M[i * LD + p + 1] * X[(p + 1) * LD + j];
-        }
-        if (p < i) a0 -= M[i * LD + p] * X[p * LD + j];
-        X[i * LD + j] = a0 + a1;
-      }
+      for (int i = j + 1; i < nb; ++i) X[i * LD + j] = -dot(i, j, i);
     } else {
       X[j * LD + j] = T(1) / M[j * LD + j];
-      for (int i = j - 1; i >= 0; --i) {
-        T a0 = T(0), a1 = T(0);
-        int p = i + 1;
-        for (; p + 1 <= j; p += 2) {
-          a0 -= M[i * LD + p] * X[p * LD + j];
-          a1 -= M[i * LD + p + 1] * X[(p + 1) * LD + j];
-        }
-        if (p <= j) a0 -= M[i * LD + p] * X[p * LD + j];
-        X[i * LD + j] = (a0 + a1) / M[i * LD + i];
-      }
+      for (int i = j - 1; i >= 0; --i) X[i * LD + j] = -dot(i, i + 1, j + 1) / M[i * LD + i];
     }
   }
   __syncthreads();
