@@ -205,6 +205,22 @@ static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a,
   // pivots by rowmap_kernel.
   bool pmap_ok[2] = {false, false};
   auto parity = [&](int k) { return (k / (int)nb) & 1; };
+  // Leaf width of the panel at column k: the widest resident leaf, or, for
+  // FP64, half of it while at least kNarrowLeafRows rows remain.  There the
+  // FP64 bulk update (MFMA-bound) sets the pace; a leaf half as wide keeps
+  // twice the rows per workgroup, so half as many CUs are reserved for it
+  // and the bulk update gets the rest (the FP32 leaf chain is the critical
+  // path at every size, and keeps its full width).
+  constexpr int kNarrowLeafRows = 12288;
+  auto leaf_width = [&](int k, int kb) -> int {
+    if (tournament) return 0;
+    int lw = rpanel_leaf_width<T>(ctx, kComplex ? std::max(kb, kLeafW) : kb, m - k);
+    if (std::is_same<T, double>::value && lw > kLeafW && m - k >= kNarrowLeafRows) {
+      const int lw2 = rpanel_leaf_width<T>(ctx, lw / 2, m - k);
+      if (lw2 > 0) lw = lw2;
+    }
+    return lw;
+  };
   // Factors the panel of columns [k, k + kb) on the context's stream.
   auto factor_panel = [&](int k, int kb) -> hipError_t {
     // Resident leaves (rows kept in LDS for the whole leaf).  A leaf narrower
@@ -212,7 +228,7 @@ static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a,
     // columns, the U rows right of it (trsm) and the update below (gemm).
     // (A complex panel narrower than the narrowest leaf goes to a leaf of
     // that width with fewer columns.)
-    const int lw = tournament ? 0 : rpanel_leaf_width<T>(ctx, kComplex ? std::max(kb, kLeafW) : kb, m - k);
+    const int lw = leaf_width(k, kb);
     const int pp = parity(k);
     pmap_ok[pp] = lw >= kb;
     if (lw > 0) {
@@ -305,12 +321,13 @@ static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a,
                       a + (int64_t)(k + kb) * lda + c0, c0, LAIR_KCLASS_GEMM);
     return e2;
   };
-  // The bulk updates leave the first (tallest) panel's workgroups their CUs:
-  // a resident panel needs all its workgroups running at once.
-  {
-    const int lw0 = tournament ? 0 : rpanel_leaf_width<T>(ctx, std::min<int>((int)nb, kmax), m);
-    ctx->panel_wgs = lw0 > 0 ? rpanel_grid<T>(ctx, lw0, m) : 0;
-  }
+  // A bulk update leaves the concurrent panel's leaf workgroups their CUs:
+  // a resident leaf needs all its workgroups running at once.
+  auto reserve_for = [&](int k) {
+    const int kb = std::min<int>((int)nb, kmax - k);
+    const int lw = k < kmax ? leaf_width(k, kb) : 0;
+    ctx->panel_wgs = lw > 0 ? rpanel_grid<T>(ctx, lw, m - k) : 0;
+  };
   // Look-ahead of depth one.  Stream 1 (the context's stream) carries the
   // critical path: panel k+1's columns are updated first, then panel k+1 is
   // factored.  Stream 2 meanwhile applies panel k to everything else: the
@@ -326,6 +343,7 @@ static lair_status getrf_impl(lair_context *ctx, int64_t m64, int64_t n64, T *a,
     const bool has_next = k + kb < kmax;
     const int nxt1 = has_next ? std::min<int>(k + kb + (int)nb, n) : k + kb;
     LAIR_HIP(hipEventRecord(ctx->ev_panel, ctx->stream));
+    reserve_for(k + kb);
     {
       SideStream side(ctx);
       LAIR_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_panel, 0));
