@@ -241,7 +241,10 @@ typedef struct lair_ir_info {
  * Stops when the normwise backward error is <= tol (tol <= 0 selects
  * n * DBL_EPSILON) or after max_iter steps; a system that does not converge is
  * re-solved with the FP64 factorization (info->fell_back = 1, status
- * LAIR_ERR_NOCONV).  A and B are not modified.  Synchronizes the stream. */
+ * LAIR_ERR_NOCONV).  A and B are not modified; X must not overlap them.
+ * Returns when all of its work on the context's stream has finished (the
+ * refinement reads its norms from pinned host memory that a kernel writes
+ * last, or synchronizes the stream). */
 lair_status lair_dgesv_mixed_device(lair_context *ctx, int64_t n, int64_t nrhs, const double *a,
                                     int64_t lda, const double *b, int64_t ldb, double *x,
                                     int64_t ldx, int32_t prec, int32_t max_iter, double tol,
