@@ -100,12 +100,13 @@ static int64_t pad_ld(int64_t n) {
 }
 
 // Default panel width.  FP64 keeps 64-column panels (one 64-column leaf, no
-// in-panel update) up to k = 8192: measured 1.94 against 2.30 ms at n=512,
-// 38.4 against 43.4 ms at n=8192; from 12288 on the 128-column panels' fewer
-// bulk passes win (78.6 against 82.3 ms at 12288, 134 against 154 at 16384).
+// in-panel update) up to k = 10240: measured 1.94 against 2.30 ms at n=512,
+// 38.4 against 43.4 ms at n=8192, 56.2 against 57.9 ms at n=10240; from 12288
+// on the 128-column panels' fewer bulk passes win (78.6 against 82.3 ms at
+// 12288, 134 against 154 ms at 16384).
 static int choose_nb(int64_t m, int64_t n, bool fp64) {
   const int64_t k = std::min(m, n);
-  if (k <= 256 || (fp64 && k <= 8192)) return 64;
+  if (k <= 256 || (fp64 && k <= 10240)) return 64;
   return 128;
 }
 
